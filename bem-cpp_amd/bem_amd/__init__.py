@@ -1,0 +1,295 @@
+"""Python front end of libbem_amd.so (the MI355X dense PMCHWT library).
+
+Every call goes through the C ABI declared in include/bem_abi.h; there is no
+CPU fallback.  If the shared library has not been built, importing this module
+raises ``RuntimeError`` -- run ``__graft_entry__.build()`` (or ``make -C
+bem-cpp_amd/csrc``) first.
+
+Complex arrays are numpy complex128 (interleaved re/im, the layout the ABI
+uses).  Right-hand sides and solutions are passed as (n, nrhs) arrays; the ABI
+stores them column-major (one right-hand side per contiguous column), as
+solver.h:13-15 of the reference does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libbem_amd.so")
+CLI_PATH = os.path.join(_PKG_ROOT, "bin", "bem_dense")
+
+F64 = 0
+F32_3M = 1
+ASM_Z64 = 1
+ASM_Z32 = 2
+ERR_NOCONV = -5
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+class BemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int), ("residual0", ctypes.c_double),
+                ("residual", ctypes.c_double), ("t_solve_ms", ctypes.c_double), ("t_ir_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _cd(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def lib():
+    """Loads libbem_amd.so (raises RuntimeError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libbem_amd.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, d, u = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint
+    sig = {
+        "bem_create": ([i], vp), "bem_destroy": ([vp], None), "bem_last_error": ([vp], ctypes.c_char_p),
+        "bem_timings": ([vp, _dp, i], i),
+        "bem_mesh_counts": ([i, i, _ip, _ip], i), "bem_make_mesh": ([i, i, d, _dp, _ip], i),
+        "bem_orientations": ([i, i, i, i, i, i, i, _dp, _dp], i),
+        "bem_set_mesh": ([vp, _dp, i, _ip, i, i], i), "bem_num_rwg": ([vp], i),
+        "bem_get_rwg": ([vp, _ip, _ip, _dp, _dp, _dp, _dp, _dp], i),
+        "bem_assemble": ([vp, d, d, d, d, d, d, u], i), "bem_get_Z": ([vp, _dp, i], i),
+        "bem_assemble_LK": ([vp, d, d, _dp, _dp], i),
+        "bem_factor": ([vp, i], i), "bem_set_Z": ([vp, _dp, i, i], i), "bem_get_pivots": ([vp, _ip], i),
+        "bem_get_lu": ([vp, _dp], i), "bem_set_lu": ([vp, _dp, _ip, i], i),
+        "bem_solve": ([vp, _dp, i, _dp, d, i, ctypes.POINTER(SolveStats)], i),
+        "bem_rhs_planewave": ([vp, d, d, d, _dp, _dp, i, _dp], i),
+        "bem_farfield": ([vp, _dp, _dp, _dp, d, d, d, i, i, i, _dp], i),
+        "bem_orient_mueller": ([vp, d, d, d, _dp, _dp, i, _dp, i, i, _dp, _dp], i),
+        "bem_mueller_grad": ([vp, _dp, _dp], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def make_mesh(shape: str, refinements: int, param: float = None):
+    """Reference mesh generators (mesh.cpp:11-154) from the library's host code."""
+    s = 1 if shape == "hex" else 0
+    if param is None:
+        param = 1.0
+    nv, nt = ctypes.c_int(), ctypes.c_int()
+    rc = lib().bem_mesh_counts(s, refinements, ctypes.byref(nv), ctypes.byref(nt))
+    if rc:
+        raise BemError(rc, "bem_mesh_counts")
+    v = np.zeros((nv.value, 3))
+    t = np.zeros((nt.value, 3), dtype=np.int32)
+    rc = lib().bem_make_mesh(s, refinements, float(param), _p(v), _pi(t))
+    if rc:
+        raise BemError(rc, "bem_make_mesh")
+    return v, t
+
+
+def orientations(na, nb, ng, beta_sym=1, gamma_sym=1, mirror=False, sort=True):
+    n = na * nb * ng
+    RT = np.zeros((n, 3, 3))
+    w = np.zeros(n)
+    got = lib().bem_orientations(na, nb, ng, beta_sym, gamma_sym, int(mirror), int(sort), _p(RT), _p(w))
+    if got < 0:
+        raise BemError(got, "bem_orientations")
+    return RT[:got], w[:got]
+
+
+class Context:
+    """One device context (mesh, Z, factors, right-hand sides, solutions)."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        self._h = self._L.bem_create(device)
+        if not self._h:
+            raise RuntimeError(f"bem_create({device}) failed: no usable GPU")
+        self.N = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.bem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what, allow=()):
+        if rc < 0 and rc not in allow:
+            raise BemError(rc, f"{what}: {self._L.bem_last_error(self._h).decode()}")
+        return rc
+
+    # -- mesh -------------------------------------------------------------
+    def set_mesh(self, verts, tris, quad_order: int = 7):
+        v = _cd(verts)
+        t = np.ascontiguousarray(tris, dtype=np.int32)
+        self._check(self._L.bem_set_mesh(self._h, _p(v), len(v), _pi(t), len(t), quad_order), "set_mesh")
+        self.N = self._L.bem_num_rwg(self._h)
+        return self.N
+
+    def rwg(self):
+        N = self.N
+        tp, tm = np.zeros(N, np.int32), np.zeros(N, np.int32)
+        fp, fm = np.zeros((N, 3)), np.zeros((N, 3))
+        ln, ap, am = np.zeros(N), np.zeros(N), np.zeros(N)
+        self._check(self._L.bem_get_rwg(self._h, _pi(tp), _pi(tm), _p(fp), _p(fm), _p(ln), _p(ap), _p(am)), "get_rwg")
+        return dict(N=N, tri_p=tp, tri_m=tm, free_p=fp, free_m=fm, length=ln, area_p=ap, area_m=am)
+
+    # -- assembly ---------------------------------------------------------
+    def assemble(self, k_ext: complex, k_int: complex, eta_ext: float, eta_int: float, z64=True, z32=True):
+        flags = (ASM_Z64 if z64 else 0) | (ASM_Z32 if z32 else 0)
+        k_ext, k_int = complex(k_ext), complex(k_int)
+        self._check(self._L.bem_assemble(self._h, k_ext.real, k_ext.imag, k_int.real, k_int.imag, eta_ext, eta_int,
+                                         flags), "assemble")
+
+    def get_Z(self):
+        n = 2 * self.N
+        Z = np.zeros((n, n), np.complex128)
+        self._check(self._L.bem_get_Z(self._h, _p(Z), 1), "get_Z")
+        return Z
+
+    def assemble_LK(self, k: complex):
+        k = complex(k)
+        L = np.zeros((self.N, self.N), np.complex128)
+        K = np.zeros((self.N, self.N), np.complex128)
+        self._check(self._L.bem_assemble_LK(self._h, k.real, k.imag, _p(L), _p(K)), "assemble_LK")
+        return L, K
+
+    # -- LU ---------------------------------------------------------------
+    def set_Z(self, Z):
+        Zc = np.ascontiguousarray(Z, dtype=np.complex128)
+        self._check(self._L.bem_set_Z(self._h, _p(Zc), Zc.shape[0], 1), "set_Z")
+
+    def factor(self, precision="fp32"):
+        p = F32_3M if precision in ("fp32", F32_3M) else F64
+        return self._check(self._L.bem_factor(self._h, p), "factor")
+
+    def get_pivots(self, n):
+        ip = np.zeros(n, np.int32)
+        self._check(self._L.bem_get_pivots(self._h, _pi(ip)), "get_pivots")
+        return ip
+
+    def get_lu(self, n):
+        LU = np.zeros((n, n), np.complex128)
+        self._check(self._L.bem_get_lu(self._h, _p(LU)), "get_lu")
+        return LU
+
+    def set_lu(self, LU, ipiv):
+        LUc = np.ascontiguousarray(LU, dtype=np.complex128)
+        ip = np.ascontiguousarray(ipiv, dtype=np.int32)
+        self._check(self._L.bem_set_lu(self._h, _p(LUc), _pi(ip), LUc.shape[0]), "set_lu")
+
+    def solve(self, B=None, ir_tol=1e-10, ir_max=10, return_X=True, n=None, nrhs=None, allow_noconv=False):
+        """Solve Z X = B.  B: (n, nrhs) or (n,) array, or None for the device RHS."""
+        st = SolveStats()
+        if B is not None:
+            B = np.asarray(B, dtype=np.complex128)
+            if B.ndim == 1:
+                B = B[:, None]
+            n, nrhs = B.shape
+            Bc = np.ascontiguousarray(B.T)
+            bp = _p(Bc)
+        else:
+            bp = None
+            if n is None:
+                n = 2 * self.N
+        X = np.zeros((nrhs, n), np.complex128) if (return_X and nrhs) else None
+        rc = self._L.bem_solve(self._h, bp, nrhs or 0, _p(X) if X is not None else None, ir_tol, ir_max,
+                               ctypes.byref(st))
+        self._check(rc, "solve", allow=(ERR_NOCONV,) if allow_noconv else ())
+        return (X.T.copy() if X is not None else None), st.as_dict()
+
+    # -- excitation / far field --------------------------------------------
+    def rhs_planewave(self, k_ext: complex, eta_ext: float, k_hat, E0, return_B=False):
+        kh = _cd(np.atleast_2d(k_hat))
+        e0 = _cd(np.atleast_2d(E0))
+        nrhs = kh.shape[0]
+        B = np.zeros((nrhs, 2 * self.N), np.complex128) if return_B else None
+        k_ext = complex(k_ext)
+        self._check(self._L.bem_rhs_planewave(self._h, k_ext.real, k_ext.imag, eta_ext, _p(kh), _p(e0), nrhs,
+                                              _p(B) if B is not None else None), "rhs_planewave")
+        return B.T.copy() if B is not None else None
+
+    def farfield(self, cJ, cM, r_hats, k_ext: complex, eta_ext: float, n_calls: int, n_orient: int):
+        cJ = np.ascontiguousarray(cJ, dtype=np.complex128)
+        cM = np.ascontiguousarray(cM, dtype=np.complex128)
+        rh = _cd(r_hats).reshape(n_orient, -1, 3)
+        ndir = rh.shape[1]
+        Fv = np.zeros((n_calls, ndir, 3), np.complex128)
+        k_ext = complex(k_ext)
+        self._check(self._L.bem_farfield(self._h, _p(cJ), _p(cM), _p(rh), k_ext.real, k_ext.imag, eta_ext, n_calls,
+                                         n_orient, ndir, _p(Fv)), "farfield")
+        return Fv
+
+    def orient_mueller(self, k_ext: complex, eta_ext: float, RT, w, theta, gamma_mirror=False, per_orient=False):
+        RTc = _cd(RT).reshape(-1, 3, 3)
+        wc = _cd(w)
+        th = _cd(theta)
+        no, nt = len(wc), len(th)
+        M = np.zeros((4, 4, nt))
+        Mo = np.zeros((no, 4, 4, nt)) if per_orient else None
+        k_ext = complex(k_ext)
+        self._check(self._L.bem_orient_mueller(self._h, k_ext.real, k_ext.imag, eta_ext, _p(RTc), _p(wc), no, _p(th),
+                                               nt, int(gamma_mirror), _p(M), _p(Mo) if Mo is not None else None),
+                    "orient_mueller")
+        return (M, Mo) if per_orient else M
+
+    def mueller_grad(self, dL_dM):
+        g = np.zeros(3)
+        d = _cd(dL_dM)
+        self._check(self._L.bem_mueller_grad(self._h, _p(d), _p(g)), "mueller_grad")
+        return g
+
+    def timings(self):
+        out = np.zeros(8)
+        self._L.bem_timings(self._h, _p(out), 8)
+        return dict(assembly_ms=out[0], factor_ms=out[1], solve_ms=out[2], refine_ms=out[3], farfield_ms=out[4],
+                    rhs_ms=out[5], adjoint_ms=out[6])
+
+
+def lab_single_orientation():
+    """RT = identity, weight 1: the --single configuration (main.cpp:672-720)."""
+    return np.eye(3)[None], np.ones(1)
+
+
+def planewave_orient_rhs(RT):
+    """k_hat = RT z, E_par = RT x, E_perp = RT y per orientation (main.cpp:740-751);
+    returns (k_hat, E0) with 2 rows per orientation (par, perp)."""
+    RT = np.asarray(RT).reshape(-1, 3, 3)
+    kh = np.repeat(RT[:, :, 2], 2, axis=0)
+    e0 = np.empty_like(kh)
+    e0[0::2] = RT[:, :, 0]
+    e0[1::2] = RT[:, :, 1]
+    return kh, e0

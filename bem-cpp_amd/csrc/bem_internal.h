@@ -1,0 +1,113 @@
+// bem_internal.h -- context layout and kernel-launcher interfaces shared by
+// the C ABI (abi.cpp) and the HIP kernels (kernels/*.hip).  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bem_abi.h"
+#include "host/bem_host.h"
+
+// Device-side geometry of one mesh.  Triangle t hosts three RWG "halves", one
+// per local edge j (vertices j, j+1; free vertex j+2).
+struct DevMesh {
+    int T = 0, N = 0, Nq = 0;
+    double4* qp = nullptr;   // [T*Nq] quadrature point (x, y, z) and jw = area * w_q
+    int4* tv = nullptr;      // [T] vertex indices (w unused)
+    double* hc = nullptr;    // [T*3] c = +-l / (2 A): f(r) = c (r - p)
+    double* hp = nullptr;    // [T*3*3] free vertex p of each half
+    int* hrwg = nullptr;     // [T*3] RWG index of each half (-1: boundary edge)
+    int4* rwg_half = nullptr;  // [N] (T+, slot+, T-, slot-)
+    int2* near = nullptr;    // [n_near] triangle pairs (t <= s) sharing a vertex
+    int n_near = 0;
+    double4* pv = nullptr;   // [T*Nq] Graglia P and V at the triangle's own points
+};
+
+// Problem constants of one assembly.
+struct AsmParams {
+    double kr[2], ki[2];  // k_ext, k_int
+    double eta[2];        // eta_ext, eta_int
+};
+
+struct bem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::string err;
+
+    // mesh and basis (host copies)
+    Mesh mesh;
+    RWG rwg;
+    TriQuad quad{};
+    bool has_mesh = false;
+    DevMesh dm;
+
+    // system matrix: n = 2N, rows/cols padded to npad, planar complex, row-major, ld = npad
+    int n = 0, npad = 0;
+    bool has_Z64 = false, has_Z32 = false;
+    double* Zr = nullptr;  // FP64 Z (kept for the refinement residual and get_Z)
+    double* Zi = nullptr;
+    float* Wr = nullptr;   // FP32 working copy / FP32 LU factors
+    float* Wi = nullptr;
+    double* Vr = nullptr;  // FP64 LU factors
+    double* Vi = nullptr;
+    int* ipiv = nullptr;   // [npad] LAPACK-style swaps (0-based row swapped with row i)
+    int* perm = nullptr;   // [npad] row permutation: row i of PZ is row perm[i] of Z
+    int factored = -1;     // -1 none, BEM_F64, BEM_F32_3M
+    AsmParams asm_params{};
+    bool asm_valid = false;
+
+    // assembly scratch: per triangle pair (t <= s) the 9 half-pair values
+    double* H = nullptr;
+    size_t H_bytes = 0;
+
+    // right-hand sides / solutions, FP64 planar row-major n x ldr
+    int nrhs = 0, ldr = 0;
+    double *Br = nullptr, *Bi = nullptr, *Xr = nullptr, *Xi = nullptr;
+    size_t rhs_cap = 0;  // doubles per plane allocated
+    bool has_B = false, has_X = false;
+
+    // far field / Mueller of the last bem_orient_mueller call (for the adjoint)
+    std::vector<double> last_RT, last_w, last_theta;
+    int last_norient = 0, last_ntheta = 0, last_mirror = 0;
+    double last_kext[2] = {0, 0}, last_eta_ext = 1.0;
+
+    // scratch allocations owned by the kernels (grown on demand)
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+
+    double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Error helpers ---------------------------------------------------------------
+int bem_fail(bem_ctx* c, int code, const std::string& msg);
+#define BEM_HIP(ctx, call)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return bem_fail((ctx), -3, std::string(#call) + ": " + hipGetErrorString(e_) + " @" +       \
+                                           __FILE__ + ":" + std::to_string(__LINE__));                  \
+    } while (0)
+
+// Device scratch of at least `bytes` (reallocated when too small).
+int ctx_scratch(bem_ctx* c, size_t bytes, void** out);
+
+// Launchers (kernels/*.hip).  All enqueue on c->stream and return 0 or < 0.
+int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32);
+int lu_factor_launch(bem_ctx* c, int precision);
+int lu_solve_launch(bem_ctx* c, int nrhs, double ir_tol, int ir_max, int* iters, double* res0, double* res,
+                    double* t_solve_ms, double* t_ir_ms);
+int rhs_launch(bem_ctx* c, double kr, double ki, double eta, const double* khat, const double* E0, int nrhs);
+int ff_launch(bem_ctx* c, const double* dJr, const double* dJi, const double* dMr, const double* dMi, int ldc,
+              const double* drh, double kr, double ki, double eta, int n_calls, int n_orient, int ndir,
+              double* dFv);
+int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* RT, const double* w, int n_orient,
+                   const double* theta, int ntheta, int mirror, double* M_avg, double* M_orient);
+int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad);
+
+// Small helpers used by several launchers
+int ctx_ensure_rhs(bem_ctx* c, int nrhs);
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }

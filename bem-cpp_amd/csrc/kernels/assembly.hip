@@ -1,0 +1,631 @@
+// assembly.hip -- PMCHWT matrix assembly on gfx950.
+//
+// Reference algorithm: assemble_LK_kernel + precompute_half + the three
+// pp/pm/mm passes + apply_singular_corrections + symmetrisation + block
+// formation (assembly.cu:14-135, 150-201, 208-321, 328-505; pmchwt.cu:6-56).
+//
+// Design.  The reference evaluates every (test RWG half, source RWG half) pair
+// on its own: 3 N^2 half pairs x Nq^2 point pairs per wavenumber.  All halves
+// on a triangle share its quadrature points, so the Green's function values of
+// a triangle pair (T, T') serve all 3 x 3 half pairs on it.  With
+//   f_a(r) = c_a (r - p_a)            (c_a = +-l/(2A), p_a free vertex)
+// the half-pair integrals are fixed combinations of 14 complex moments of the
+// triangle pair (W_ij = jw_i jw_j, G = e^{ikR}/(4 pi R), g = G (ik - 1/R)/R):
+//   M0 = sum W G,  Mr = sum W G r_i,  Ms = sum W G r_j,  Mrs = sum W G r_i.r_j
+//   Nx = sum W g r_i x r_j,           Nd = sum W g (r_i - r_j)
+//   L_ab = c_a c_b [ ik (Mrs - p_a.Ms - p_b.Mr + (p_a.p_b) M0) - (4i/k) M0 ]
+//   K_ab = c_a c_b [ (p_b - p_a).Nx + (p_b x p_a).Nd ]
+// (the K identity: (r_i-p_a).((r_i-r_j) x (r_j-p_b)) = (p_b-p_a).(r_i x r_j)
+//  + (p_b x p_a).(r_i - r_j)).  One pass over the T(T+1)/2 triangle pairs,
+// Nq^2 point pairs each, both wavenumbers sharing the geometry, replaces the
+// reference's 6 N^2 Nq^2 half-pair evaluations (13.5x fewer point pairs).
+//
+// Precision.  Distances and phases are formed in FP64 (FP32 rsqrt seed + one
+// FP64 Newton step), the phase is reduced to a quarter turn in FP64, and only
+// sin/cos/exp of the reduced argument run in FP32; all moments accumulate in
+// FP64 (their expansion about the origin is exact for FP64 sums of the same
+// FP32-rounded G values).  Triangle pairs that share a vertex (self and
+// adjacent: singular and near-singular integrands) run entirely in FP64 in
+// their own kernel, which also adds the Graglia static part for self pairs.
+//
+// Kernels, in launch order:
+//   k_graglia      P(r_q), V(r_q) at every triangle's own points      (FP64)
+//   k_tripair_far  non-touching triangle pairs -> H records   (FP32 trig, FP64 sums)
+//   k_tripair_near touching pairs incl. self    -> H records   (FP64)
+//   k_gather       Z[m][n] = sum of the 4 half-pair records of (m, n) in a
+//                  fixed order; PMCHWT blocks formed in place, FP64 and FP32
+//   k_pad          identity on the padding rows/columns
+// H record of pair (t <= s): 9 half pairs (a on t, b on s) x {P, Q, Ks} complex,
+// P = eta_e L_e + eta_i L_i, Q = L_e/eta_e + L_i/eta_i, Ks = K_e + K_i.
+// No atomics: every record has one writer, every Z entry one reader-summer.
+#include "../bem_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kInv4Pi = 1.0 / (4.0 * kPi);
+constexpr int kRec = 54;  // doubles per H record
+
+__device__ __forceinline__ long long rec_index(int t, int s, int T) {
+    // packed upper triangle; row t holds s = t..T-1
+    return (long long)t * T - (long long)t * (t - 1) / 2 + (s - t);
+}
+
+// sin/cos on [-pi/4, pi/4]: Taylor series to x^9 / x^10 (truncation < 2e-9).
+__device__ __forceinline__ void sincos_poly(float x, float& s, float& c) {
+    const float x2 = x * x;
+    s = x * fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 2.75573192e-6f, -1.98412698e-4f), 8.33333333e-3f),
+                          -1.66666667e-1f), 1.0f);
+    c = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, -2.75573192e-7f, 2.48015873e-5f), -1.38888889e-3f),
+                               4.16666667e-2f), -0.5f), 1.0f);
+}
+
+// sin and cos of a phase of u >= 0 quarter turns, reduced in FP64.
+__device__ __forceinline__ void sincos_quarter(double u, float& sn, float& cs) {
+    const double q = rint(u);
+    const float x = (float)((u - q) * (0.5 * kPi));
+    const int iq = (int)q & 3;
+    float s, c;
+    sincos_poly(x, s, c);
+    const bool swap = iq & 1;
+    const float sv = swap ? c : s, cv = swap ? s : c;
+    sn = (iq & 2) ? -sv : sv;
+    cs = ((iq + 1) & 2) ? -cv : cv;
+}
+
+// e^y for y <= 0 from an FP64 argument: y = n ln2 + r, |r| <= ln2/2, e^r by its
+// Taylor series to r^8 (truncation < 6e-9), scaled by 2^n.
+__device__ __forceinline__ float exp_neg(double y) {
+    const double n = rint(y * 1.4426950408889634);
+    const float r = (float)(y - n * 0.6931471805599453);
+    float p = fmaf(r, 2.48015873e-5f, 1.98412698e-4f);
+    p = fmaf(r, p, 1.38888889e-3f);
+    p = fmaf(r, p, 8.33333333e-3f);
+    p = fmaf(r, p, 4.16666667e-2f);
+    p = fmaf(r, p, 1.66666667e-1f);
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r, p, 1.0f);
+    p = fmaf(r, p, 1.0f);
+    return ldexpf(p, (int)n);
+}
+
+struct C2 {
+    double r, i;
+};
+
+// Moments of one triangle pair for one wavenumber ([..][0] real, [..][1] imag).
+struct Mom {
+    double m0[2], mr[3][2], ms[3][2], mrs[2], nx[3][2], nd[3][2];
+};
+
+__device__ __forceinline__ void mom_zero(Mom& m) {
+    m.m0[0] = m.m0[1] = m.mrs[0] = m.mrs[1] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        m.mr[d][0] = m.mr[d][1] = m.ms[d][0] = m.ms[d][1] = m.nx[d][0] = m.nx[d][1] = m.nd[d][0] = m.nd[d][1] = 0.0;
+}
+
+// Inner sums over source points j for one test point i:
+// a = sum w_j G, b = sum w_j G r_j, d = sum w_j g, c = sum w_j g r_j.
+struct Inner {
+    double a[2], b[3][2], d[2], c[3][2];
+};
+
+__device__ __forceinline__ void inner_zero(Inner& s) {
+    s.a[0] = s.a[1] = s.d[0] = s.d[1] = 0.0;
+#pragma unroll
+    for (int e = 0; e < 3; e++) s.b[e][0] = s.b[e][1] = s.c[e][0] = s.c[e][1] = 0.0;
+}
+
+__device__ __forceinline__ void inner_add(Inner& s, double w, double Gr, double Gi, double gr, double gi,
+                                          const double4& q) {
+    const double wGr = w * Gr, wGi = w * Gi, wgr = w * gr, wgi = w * gi;
+    s.a[0] += wGr;
+    s.a[1] += wGi;
+    s.d[0] += wgr;
+    s.d[1] += wgi;
+    s.b[0][0] = fma(wGr, q.x, s.b[0][0]); s.b[0][1] = fma(wGi, q.x, s.b[0][1]);
+    s.b[1][0] = fma(wGr, q.y, s.b[1][0]); s.b[1][1] = fma(wGi, q.y, s.b[1][1]);
+    s.b[2][0] = fma(wGr, q.z, s.b[2][0]); s.b[2][1] = fma(wGi, q.z, s.b[2][1]);
+    s.c[0][0] = fma(wgr, q.x, s.c[0][0]); s.c[0][1] = fma(wgi, q.x, s.c[0][1]);
+    s.c[1][0] = fma(wgr, q.y, s.c[1][0]); s.c[1][1] = fma(wgi, q.y, s.c[1][1]);
+    s.c[2][0] = fma(wgr, q.z, s.c[2][0]); s.c[2][1] = fma(wgi, q.z, s.c[2][1]);
+}
+
+// Folds the inner sums of test point p (weight p.w) into the moments.
+__device__ __forceinline__ void mom_fold(Mom& m, const Inner& s, const double4& p) {
+    const double w = p.w;
+    const double r[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        m.m0[c] = fma(w, s.a[c], m.m0[c]);
+        double dotb = 0.0;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            m.mr[e][c] = fma(w * r[e], s.a[c], m.mr[e][c]);
+            m.ms[e][c] = fma(w, s.b[e][c], m.ms[e][c]);
+            dotb = fma(r[e], s.b[e][c], dotb);
+            m.nd[e][c] = fma(w, r[e] * s.d[c] - s.c[e][c], m.nd[e][c]);
+        }
+        m.mrs[c] = fma(w, dotb, m.mrs[c]);
+        m.nx[0][c] = fma(w, r[1] * s.c[2][c] - r[2] * s.c[1][c], m.nx[0][c]);
+        m.nx[1][c] = fma(w, r[2] * s.c[0][c] - r[0] * s.c[2][c], m.nx[1][c]);
+        m.nx[2][c] = fma(w, r[0] * s.c[1][c] - r[1] * s.c[0][c], m.nx[2][c]);
+    }
+}
+
+// L_ab and K_ab from the moments (formulas in the file header).
+__device__ __forceinline__ void half_pair(const Mom& m, double kr, double ki, double ca, const double* pa, double cb,
+                                          const double* pb, C2& L, C2& K) {
+    const double papb = pa[0] * pb[0] + pa[1] * pb[1] + pa[2] * pb[2];
+    const double cr[3] = {pb[1] * pa[2] - pb[2] * pa[1], pb[2] * pa[0] - pb[0] * pa[2],
+                          pb[0] * pa[1] - pb[1] * pa[0]};  // p_b x p_a
+    double lm[2], km[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        double v = m.mrs[c] + papb * m.m0[c];
+        double k = 0.0;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            v -= pa[e] * m.ms[e][c] + pb[e] * m.mr[e][c];
+            k += (pb[e] - pa[e]) * m.nx[e][c] + cr[e] * m.nd[e][c];
+        }
+        lm[c] = v;
+        km[c] = k;
+    }
+    // ik * lm - (4i/k) m0, with i/k = (ki + i kr)/|k|^2
+    const double k2 = kr * kr + ki * ki;
+    const double fr = 4.0 * ki / k2, fi = 4.0 * kr / k2;
+    const double Lr = -ki * lm[0] - kr * lm[1] - (fr * m.m0[0] - fi * m.m0[1]);
+    const double Li = -ki * lm[1] + kr * lm[0] - (fr * m.m0[1] + fi * m.m0[0]);
+    const double cc = ca * cb;
+    L.r = cc * Lr;
+    L.i = cc * Li;
+    K.r = cc * km[0];
+    K.i = cc * km[1];
+}
+
+// Writes the 9 half-pair values of pair (t, s); `corr` (self pairs only)
+// holds the symmetrised static part per wavenumber and half pair.
+__device__ __forceinline__ void write_record(double* __restrict__ H, long long rec, const Mom* mom,
+                                             const AsmParams& p, const double* __restrict__ hc,
+                                             const double* __restrict__ hp, int t, int s, const C2* corr) {
+    double* out = H + rec * kRec;
+#pragma unroll 1
+    for (int a = 0; a < 3; a++) {
+        const double ca = hc[t * 3 + a];
+        const double pa[3] = {hp[(t * 3 + a) * 3], hp[(t * 3 + a) * 3 + 1], hp[(t * 3 + a) * 3 + 2]};
+#pragma unroll 1
+        for (int b = 0; b < 3; b++) {
+            const double cb = hc[s * 3 + b];
+            const double pb[3] = {hp[(s * 3 + b) * 3], hp[(s * 3 + b) * 3 + 1], hp[(s * 3 + b) * 3 + 2]};
+            C2 L[2], K[2];
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                half_pair(mom[kk], p.kr[kk], p.ki[kk], ca, pa, cb, pb, L[kk], K[kk]);
+                if (corr) {
+                    L[kk].r += corr[kk * 9 + a * 3 + b].r;
+                    L[kk].i += corr[kk * 9 + a * 3 + b].i;
+                }
+            }
+            double2* o = reinterpret_cast<double2*>(out + (a * 3 + b) * 6);
+            o[0] = make_double2(p.eta[0] * L[0].r + p.eta[1] * L[1].r, p.eta[0] * L[0].i + p.eta[1] * L[1].i);
+            o[1] = make_double2(L[0].r / p.eta[0] + L[1].r / p.eta[1], L[0].i / p.eta[0] + L[1].i / p.eta[1]);
+            o[2] = make_double2(K[0].r + K[1].r, K[0].i + K[1].i);
+        }
+    }
+}
+
+__device__ __forceinline__ bool touching(const int4& a, const int4& b) {
+    return a.x == b.x || a.x == b.y || a.x == b.z || a.y == b.x || a.y == b.y || a.y == b.z || a.z == b.x ||
+           a.z == b.y || a.z == b.z;
+}
+
+// ---------------------------------------------------------------------------
+// Non-touching triangle pairs.  Block = 16 x 16 triangles (t in the row tile,
+// s in the column tile), only tiles with row <= column; thread (ti, tj).
+// ---------------------------------------------------------------------------
+template <int NQ, bool LOSSY>
+__global__ __launch_bounds__(256) void k_tripair_far(int T, const double4* __restrict__ qp,
+                                                     const int4* __restrict__ tv, const double* __restrict__ hc,
+                                                     const double* __restrict__ hp, AsmParams p,
+                                                     double* __restrict__ H) {
+    const int bt = blockIdx.y, bs = blockIdx.x;
+    if (bt > bs) return;
+    __shared__ double4 sT[16 * NQ], sS[16 * NQ];
+    __shared__ int4 vT[16], vS[16];
+    for (int i = threadIdx.x; i < 16 * NQ; i += 256) {
+        const int t = bt * 16 + i / NQ, s = bs * 16 + i / NQ;
+        sT[i] = t < T ? qp[t * NQ + i % NQ] : make_double4(0, 0, 0, 0);
+        sS[i] = s < T ? qp[s * NQ + i % NQ] : make_double4(0, 0, 0, 0);
+    }
+    if (threadIdx.x < 16) {
+        const int t = bt * 16 + threadIdx.x, s = bs * 16 + threadIdx.x;
+        vT[threadIdx.x] = t < T ? tv[t] : make_int4(-1, -1, -1, 0);
+        vS[threadIdx.x] = s < T ? tv[s] : make_int4(-2, -2, -2, 0);
+    }
+    __syncthreads();
+    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+    const int t = bt * 16 + ti, s = bs * 16 + tj;
+    if (t >= T || s >= T || t > s) return;
+    if (touching(vT[ti], vS[tj])) return;  // k_tripair_near owns this record
+
+    const double kq0 = p.kr[0] * (2.0 / kPi), kq1 = p.kr[1] * (2.0 / kPi);  // quarter turns per unit length
+    const float kif0 = (float)p.ki[0], kif1 = (float)p.ki[1];
+    const float krf0 = (float)p.kr[0], krf1 = (float)p.kr[1];
+    const bool lossy0 = p.ki[0] != 0.0;
+    Mom mom[2];
+    mom_zero(mom[0]);
+    mom_zero(mom[1]);
+#pragma unroll 1
+    for (int i = 0; i < NQ; i++) {
+        const double4 P = sT[ti * NQ + i];
+        Inner in0, in1;
+        inner_zero(in0);
+        inner_zero(in1);
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const double4 Q = sS[tj * NQ + j];
+            const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            double y = (double)__builtin_amdgcn_rsqf((float)r2);
+            y = y * fma(-0.5 * r2, y * y, 1.5);  // Newton step in FP64
+            const double R = r2 * y;
+            const float invR = (float)y;
+            const float amp = invR * (float)kInv4Pi;
+            {  // exterior wavenumber
+                float sn, cs;
+                sincos_quarter(R * kq0, sn, cs);
+                const float a0 = lossy0 ? amp * exp_neg(-p.ki[0] * R) : amp;
+                const float Gr = a0 * cs, Gi = a0 * sn;
+                const float ar = (-kif0 - invR) * invR, ai = krf0 * invR;
+                inner_add(in0, Q.w, Gr, Gi, Gr * ar - Gi * ai, Gr * ai + Gi * ar, Q);
+            }
+            {  // interior wavenumber
+                float sn, cs;
+                sincos_quarter(R * kq1, sn, cs);
+                const float a1 = LOSSY ? amp * exp_neg(-p.ki[1] * R) : amp;
+                const float Gr = a1 * cs, Gi = a1 * sn;
+                const float ar = (-kif1 - invR) * invR, ai = krf1 * invR;
+                inner_add(in1, Q.w, Gr, Gi, Gr * ar - Gi * ai, Gr * ai + Gi * ar, Q);
+            }
+        }
+        mom_fold(mom[0], in0, P);
+        mom_fold(mom[1], in1, P);
+    }
+    write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Graglia integrals at every quadrature point of its own triangle (FP64):
+// P(r) = int_T 1/|r - r'| (graglia.h:11-66) and V(r) = r P - grad W with
+// W(r) = A sum_q w_q |r_q - r| differentiated by central differences of step
+// 1e-6 (graglia.h:69-112).  One thread per (triangle, point).
+// ---------------------------------------------------------------------------
+__device__ double pot_integral(const double r[3], const double v[3][3]) {
+    const double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nn < 1e-15) return 0.0;
+    n[0] *= 1.0 / nn;
+    n[1] *= 1.0 / nn;
+    n[2] *= 1.0 / nn;
+    double res = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const double* a = v[i];
+        const double* b = v[(i + 1) % 3];
+        double t[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double le = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        if (le < 1e-15) continue;
+        t[0] *= 1.0 / le;
+        t[1] *= 1.0 / le;
+        t[2] *= 1.0 / le;
+        const double m[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+        const double ra[3] = {r[0] - a[0], r[1] - a[1], r[2] - a[2]};
+        const double d = ra[0] * m[0] + ra[1] * m[1] + ra[2] * m[2];
+        const double h = ra[0] * n[0] + ra[1] * n[1] + ra[2] * n[2];
+        const double bj[3] = {b[0] - r[0], b[1] - r[1], b[2] - r[2]};
+        const double ai[3] = {a[0] - r[0], a[1] - r[1], a[2] - r[2]};
+        const double sp = bj[0] * t[0] + bj[1] * t[1] + bj[2] * t[2];
+        const double sm = ai[0] * t[0] + ai[1] * t[1] + ai[2] * t[2];
+        const double Rp = sqrt(bj[0] * bj[0] + bj[1] * bj[1] + bj[2] * bj[2]);
+        const double Rm = sqrt(ai[0] * ai[0] + ai[1] * ai[1] + ai[2] * ai[2]);
+        if (Rp + sp > 1e-15 && Rm + sm > 1e-15) {
+            const double la = (Rp + sp) / (Rm + sm);
+            if (la > 0) res += d * log(la);
+        }
+        if (fabs(h) > 1e-15) {
+            const double R0 = d * d + h * h;
+            res -= fabs(h) * (atan2(d * sp, R0 + fabs(h) * Rp) - atan2(d * sm, R0 + fabs(h) * Rm));
+        }
+    }
+    return res;
+}
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_graglia(int T, const double4* __restrict__ qp,
+                                                 const int4* __restrict__ tv, const double* __restrict__ verts,
+                                                 const double* __restrict__ qw, double4* __restrict__ pv) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= T * NQ) return;
+    const int t = idx / NQ;
+    const int4 iv = tv[t];
+    const int ids[3] = {iv.x, iv.y, iv.z};
+    double v[3][3];
+    for (int k = 0; k < 3; k++)
+        for (int e = 0; e < 3; e++) v[k][e] = verts[ids[k] * 3 + e];
+    const double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                 cz = e1[0] * e2[1] - e1[1] * e2[0];
+    const double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+    const double4 q = qp[idx];
+    const double r[3] = {q.x, q.y, q.z};
+    const double P = pot_integral(r, v);
+    const double eps = 1e-6;
+    double g[3];
+    for (int d = 0; d < 3; d++) {
+        double rp[3] = {r[0], r[1], r[2]}, rm[3] = {r[0], r[1], r[2]};
+        rp[d] += eps;
+        rm[d] -= eps;
+        double wp = 0.0, wm = 0.0;
+        for (int j = 0; j < NQ; j++) {
+            const double4 s = qp[t * NQ + j];
+            const double ap[3] = {s.x - rp[0], s.y - rp[1], s.z - rp[2]};
+            const double am[3] = {s.x - rm[0], s.y - rm[1], s.z - rm[2]};
+            wp += qw[j] * sqrt(ap[0] * ap[0] + ap[1] * ap[1] + ap[2] * ap[2]);
+            wm += qw[j] * sqrt(am[0] * am[0] + am[1] * am[1] + am[2] * am[2]);
+        }
+        g[d] = (area * wp - area * wm) / (2.0 * eps);
+    }
+    pv[idx] = make_double4(P, r[0] * P - g[0], r[1] * P - g[1], r[2] * P - g[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Touching triangle pairs (shared vertex, shared edge, or the same triangle),
+// all in FP64.  Self pairs use G - 1/(4 pi R) with the R = 0 value
+// i Re(k)/(4 pi) of the reference (assembly.cu:63-77), skip K (assembly.cu:94),
+// and add the symmetrised Graglia static part (assembly.cu:290-317):
+//   C_ab = c_a c_b / (4 pi) [ -(4i/k) S_P + ik (S_rV - (p_a+p_b).(S_Pr+S_V)/2
+//                                                 + (p_a.p_b) S_P) ]
+// with S_P = sum jw P, S_Pr = sum jw P r, S_V = sum jw V, S_rV = sum jw r.V.
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const int2* __restrict__ pairs,
+                                                      const double4* __restrict__ qp,
+                                                      const double* __restrict__ hc,
+                                                      const double* __restrict__ hp,
+                                                      const double4* __restrict__ pv, AsmParams p,
+                                                      double* __restrict__ H) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_near) return;
+    const int t = pairs[idx].x, s = pairs[idx].y;
+    const bool self = t == s;
+    Mom mom[2];
+    mom_zero(mom[0]);
+    mom_zero(mom[1]);
+#pragma unroll 1
+    for (int i = 0; i < NQ; i++) {
+        const double4 P = qp[t * NQ + i];
+        Inner in[2];
+        inner_zero(in[0]);
+        inner_zero(in[1]);
+#pragma unroll 1
+        for (int j = 0; j < NQ; j++) {
+            const double4 Q = qp[s * NQ + j];
+            const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double R = sqrt(r2);
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                const double kr = p.kr[kk], ki = p.ki[kk];
+                double Gr, Gi, gr = 0.0, gi = 0.0;
+                if (self && R < 1e-12) {
+                    Gr = 0.0;
+                    Gi = kr * kInv4Pi;
+                } else {
+                    double sn, cs;
+                    sincos(kr * R, &sn, &cs);
+                    const double amp = exp(-ki * R) * kInv4Pi / R;
+                    Gr = amp * cs;
+                    Gi = amp * sn;
+                    if (!self && R > 1e-6) {
+                        const double ar = (-ki - 1.0 / R) / R, ai = kr / R;
+                        gr = Gr * ar - Gi * ai;
+                        gi = Gr * ai + Gi * ar;
+                    }
+                    if (self) Gr -= kInv4Pi / R;
+                }
+                inner_add(in[kk], Q.w, Gr, Gi, gr, gi, Q);
+            }
+        }
+        mom_fold(mom[0], in[0], P);
+        mom_fold(mom[1], in[1], P);
+    }
+    if (!self) {
+        write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+        return;
+    }
+    double SP = 0.0, SPr[3] = {0, 0, 0}, SV[3] = {0, 0, 0}, SrV = 0.0;
+    for (int q = 0; q < NQ; q++) {
+        const double4 x = qp[t * NQ + q];
+        const double4 w = pv[t * NQ + q];  // (P, Vx, Vy, Vz)
+        SP += x.w * w.x;
+        SPr[0] += x.w * w.x * x.x;
+        SPr[1] += x.w * w.x * x.y;
+        SPr[2] += x.w * w.x * x.z;
+        SV[0] += x.w * w.y;
+        SV[1] += x.w * w.z;
+        SV[2] += x.w * w.w;
+        SrV += x.w * (x.x * w.y + x.y * w.z + x.z * w.w);
+    }
+    C2 corr[18];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            const double ca = hc[t * 3 + a], cb = hc[t * 3 + b];
+            const double* pa = hp + (t * 3 + a) * 3;
+            const double* pb = hp + (t * 3 + b) * 3;
+            double sym = 0.0, papb = 0.0;
+            for (int e = 0; e < 3; e++) {
+                sym += (pa[e] + pb[e]) * (SPr[e] + SV[e]);
+                papb += pa[e] * pb[e];
+            }
+            const double vec = SrV - 0.5 * sym + papb * SP;  // real
+            for (int kk = 0; kk < 2; kk++) {
+                const double kr = p.kr[kk], ki = p.ki[kk], k2 = kr * kr + ki * ki;
+                // -(4i/k) SP + ik vec, i/k = (ki + i kr)/|k|^2, ik = -ki + i kr
+                const double re = -4.0 * ki / k2 * SP - ki * vec;
+                const double im = -4.0 * kr / k2 * SP + kr * vec;
+                const double f = ca * cb * kInv4Pi;
+                corr[kk * 9 + a * 3 + b].r = f * re;
+                corr[kk * 9 + a * 3 + b].i = f * im;
+            }
+        }
+    write_record(H, rec_index(t, t, T), mom, p, hc, hp, t, t, corr);
+}
+
+// ---------------------------------------------------------------------------
+// Z[m][n] from the four half-pair records of RWGs m and n, PMCHWT blocks:
+// Z[m][n] = P, Z[m][N+n] = -Ks, Z[N+m][n] = Ks, Z[N+m][N+n] = Q.
+// One thread per (m, n); n fastest for coalesced row writes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather(int N, int T, int ld, const int4* __restrict__ rwg_half,
+                                                const double* __restrict__ H, double* __restrict__ Zr,
+                                                double* __restrict__ Zi, float* __restrict__ Wr,
+                                                float* __restrict__ Wi) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (n >= N) return;
+    const int4 hm = rwg_half[m], hn = rwg_half[n];
+    const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
+    const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
+    double P0 = 0, P1 = 0, Q0 = 0, Q1 = 0, K0 = 0, K1 = 0;
+#pragma unroll
+    for (int sa = 0; sa < 2; sa++)
+#pragma unroll
+        for (int sb = 0; sb < 2; sb++) {
+            const int ta = tm[sa], tb = tn[sb];
+            long long rec;
+            int slot;
+            if (ta <= tb) {
+                rec = rec_index(ta, tb, T);
+                slot = lm[sa] * 3 + ln[sb];
+            } else {
+                rec = rec_index(tb, ta, T);
+                slot = ln[sb] * 3 + lm[sa];
+            }
+            const double2* h = reinterpret_cast<const double2*>(H + rec * kRec + slot * 6);
+            const double2 a = h[0], b = h[1], c = h[2];
+            P0 += a.x; P1 += a.y;
+            Q0 += b.x; Q1 += b.y;
+            K0 += c.x; K1 += c.y;
+        }
+    const size_t r0 = (size_t)m * ld, r1 = (size_t)(N + m) * ld;
+    if (Zr) {
+        Zr[r0 + n] = P0;      Zi[r0 + n] = P1;
+        Zr[r0 + N + n] = -K0; Zi[r0 + N + n] = -K1;
+        Zr[r1 + n] = K0;      Zi[r1 + n] = K1;
+        Zr[r1 + N + n] = Q0;  Zi[r1 + N + n] = Q1;
+    }
+    if (Wr) {
+        Wr[r0 + n] = (float)P0;      Wi[r0 + n] = (float)P1;
+        Wr[r0 + N + n] = (float)-K0; Wi[r0 + N + n] = (float)-K1;
+        Wr[r1 + n] = (float)K0;      Wi[r1 + n] = (float)K1;
+        Wr[r1 + N + n] = (float)Q0;  Wi[r1 + N + n] = (float)Q1;
+    }
+}
+
+// Identity on rows/columns [n, ld): keeps the padded system block-diagonal.
+template <class F>
+__global__ void k_pad(int n, int ld, F* __restrict__ Ar, F* __restrict__ Ai) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= ld) return;
+    if (i < n && j < n) return;
+    Ar[(size_t)i * ld + j] = (i == j) ? F(1) : F(0);
+    Ai[(size_t)i * ld + j] = F(0);
+}
+
+template <int NQ>
+int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi, float* Wr, float* Wi,
+              const double* dverts, const double* dqw) {
+    const DevMesh& d = c->dm;
+    const int T = d.T;
+    hipLaunchKernelGGL(k_graglia<NQ>, dim3((T * NQ + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv, dverts,
+                       dqw, d.pv);
+    const int nb = (T + 15) / 16;
+    if (lossy)
+        hipLaunchKernelGGL((k_tripair_far<NQ, true>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv, d.hc,
+                           d.hp, p, c->H);
+    else
+        hipLaunchKernelGGL((k_tripair_far<NQ, false>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv,
+                           d.hc, d.hp, p, c->H);
+    if (d.n_near > 0)
+        hipLaunchKernelGGL(k_tripair_near<NQ>, dim3((d.n_near + 127) / 128), dim3(128), 0, c->stream, T, d.n_near,
+                           d.near, d.qp, d.hc, d.hp, d.pv, p, c->H);
+    const int N = d.N;
+    hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256, N), dim3(256), 0, c->stream, N, T, c->npad, d.rwg_half, c->H,
+                       Zr, Zi, Wr, Wi);
+    const int n = 2 * N;
+    if (c->npad > n) {
+        if (Zr)
+            hipLaunchKernelGGL(k_pad<double>, dim3((c->npad + 255) / 256, c->npad), dim3(256), 0, c->stream, n,
+                               c->npad, Zr, Zi);
+        if (Wr)
+            hipLaunchKernelGGL(k_pad<float>, dim3((c->npad + 255) / 256, c->npad), dim3(256), 0, c->stream, n,
+                               c->npad, Wr, Wi);
+    }
+    return 0;
+}
+
+}  // namespace
+
+// Device copies of the vertex coordinates and quadrature weights used by
+// k_graglia live in the context scratch area.
+int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
+    const DevMesh& d = c->dm;
+    const size_t T = (size_t)d.T;
+    const size_t need = T * (T + 1) / 2 * kRec * sizeof(double);
+    if (c->H_bytes < need) {
+        if (c->H) (void)hipFree(c->H);
+        c->H = nullptr;
+        c->H_bytes = 0;
+        if (hipMalloc(&c->H, need) != hipSuccess) {
+            c->H = nullptr;
+            return bem_fail(c, BEM_ERR_NOMEM, "assembly scratch: cannot allocate " + std::to_string(need) + " bytes");
+        }
+        c->H_bytes = need;
+    }
+    // vertex coordinates + weights for k_graglia
+    const int nv = c->mesh.nv();
+    const size_t sbytes = (size_t)(nv * 3 + 32) * sizeof(double);
+    void* s = nullptr;
+    if (int rc = ctx_scratch(c, sbytes, &s)) return rc;
+    std::vector<double> host(nv * 3 + 32, 0.0);
+    for (int i = 0; i < nv; i++) {
+        host[3 * i] = c->mesh.verts[i].x;
+        host[3 * i + 1] = c->mesh.verts[i].y;
+        host[3 * i + 2] = c->mesh.verts[i].z;
+    }
+    for (int q = 0; q < c->quad.npts; q++) host[nv * 3 + q] = c->quad.wts[q];
+    BEM_HIP(c, hipMemcpyAsync(s, host.data(), sbytes, hipMemcpyHostToDevice, c->stream));
+    const double* dverts = static_cast<const double*>(s);
+    const double* dqw = dverts + nv * 3;
+    const bool lossy = p.ki[1] != 0.0;
+    double* Zr = want64 ? c->Zr : nullptr;
+    double* Zi = want64 ? c->Zi : nullptr;
+    float* Wr = want32 ? c->Wr : nullptr;
+    float* Wi = want32 ? c->Wi : nullptr;
+    int rc;
+    switch (d.Nq) {
+        case 4: rc = launch_nq<4>(c, p, lossy, Zr, Zi, Wr, Wi, dverts, dqw); break;
+        case 13: rc = launch_nq<13>(c, p, lossy, Zr, Zi, Wr, Wi, dverts, dqw); break;
+        default: rc = launch_nq<7>(c, p, lossy, Zr, Zi, Wr, Wi, dverts, dqw); break;
+    }
+    if (rc) return rc;
+    BEM_HIP(c, hipGetLastError());
+    // the host staging vector must outlive the copy
+    BEM_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+}

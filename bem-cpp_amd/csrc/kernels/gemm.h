@@ -1,0 +1,227 @@
+// gemm.h -- complex GEMM on CDNA4 matrix cores, planar operands.
+//
+//   C = C - A B   (mode 0)     C = A B   (mode 1)     C = C + A B   (mode 2)
+//
+// A is M x K, B is K x N, C is M x N; every matrix is stored as two planes
+// (real, imaginary) in row-major order with its own leading dimension.
+//
+// FP32 uses the 3M form: three real products T1 = Ar Br, T2 = Ai Bi,
+// T3 = (Ar + Ai)(Br + Bi) accumulate on v_mfma_f32_16x16x4f32, and the
+// epilogue forms Re = T1 - T2, Im = T3 - T1 - T2 (3/4 of the real flops of the
+// 4M form).  FP64 uses the 4M form on v_mfma_f64_16x16x4f64 with two
+// accumulators (Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br).
+//
+// Tiling: 256 threads = 4 waves of 64 in a 2 x 2 arrangement; a K slice of BK
+// is staged through LDS (k-major, double buffered, one barrier per slice);
+// each wave owns a WM x WN sub-tile made of 16 x 16 MFMA tiles.
+// MFMA 16x16x4 operand layout (wave64): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k.  Result D[i][j]: f32 form in lane j + 16 (i / 4), register
+// i % 4; f64 form in lane j + 16 (i % 4), register i / 4 (measured on gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma16x16x4(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+struct GemmCfg;
+template <>
+struct GemmCfg<float> {
+    static constexpr int BM = 128, BN = 128, BK = 16, WM = 64, WN = 64, PAD = 16;
+    typedef v4f acc_t;
+    typedef float4 vec_t;  // 16-byte global loads
+    static constexpr int VEC = 4;
+};
+template <>
+struct GemmCfg<double> {
+    static constexpr int BM = 128, BN = 64, BK = 16, WM = 64, WN = 32, PAD = 8;
+    typedef v4d acc_t;
+    typedef double2 vec_t;
+    static constexpr int VEC = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ void vec_split(const typename GemmCfg<T>::vec_t& v, T* out);
+template <>
+__device__ __forceinline__ void vec_split<float>(const float4& v, float* o) {
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+template <>
+__device__ __forceinline__ void vec_split<double>(const double2& v, double* o) {
+    o[0] = v.x; o[1] = v.y;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cgemm(int M, int N, int K, const T* __restrict__ Ar,
+                                               const T* __restrict__ Ai, long lda, const T* __restrict__ Br,
+                                               const T* __restrict__ Bi, long ldb, T* __restrict__ Cr,
+                                               T* __restrict__ Ci, long ldc, int mode) {
+    typedef GemmCfg<T> C;
+    constexpr int BM = C::BM, BN = C::BN, BK = C::BK, WM = C::WM, WN = C::WN, VEC = C::VEC;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int SA = BM + C::PAD, SB = BN + C::PAD;
+    constexpr int AV = BM * BK / VEC / 256;  // 16-byte vectors per thread per plane
+    constexpr int BV = BK * BN / VEC / 256;
+    typedef typename C::vec_t vec_t;
+    typedef typename C::acc_t acc_t;
+
+    __shared__ T sA[2][2][BK][SA];
+    __shared__ T sB[2][2][BK][SB];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+    const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+
+    vec_t ga[2][AV], gb[2][BV];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < AV; u++) {
+            const int v = tid + 256 * u;
+            const int r = v / (BK / VEC), kv = (v % (BK / VEC)) * VEC;
+            const bool ok = row0 + r < M;
+            const long o = (long)(row0 + r) * lda + k0 + kv;
+            ga[0][u] = ok ? *reinterpret_cast<const vec_t*>(Ar + o) : vec_t{};
+            ga[1][u] = ok ? *reinterpret_cast<const vec_t*>(Ai + o) : vec_t{};
+        }
+#pragma unroll
+        for (int u = 0; u < BV; u++) {
+            const int v = tid + 256 * u;
+            const int kr = v / (BN / VEC), nv = (v % (BN / VEC)) * VEC;
+            const bool ok = col0 + nv < N;
+            const long o = (long)(k0 + kr) * ldb + col0 + nv;
+            gb[0][u] = ok ? *reinterpret_cast<const vec_t*>(Br + o) : vec_t{};
+            gb[1][u] = ok ? *reinterpret_cast<const vec_t*>(Bi + o) : vec_t{};
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < AV; u++) {
+            const int v = tid + 256 * u;
+            const int r = v / (BK / VEC), kv = (v % (BK / VEC)) * VEC;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                T e[VEC];
+                vec_split<T>(ga[p][u], e);
+#pragma unroll
+                for (int q = 0; q < VEC; q++) sA[buf][p][kv + q][r] = e[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BV; u++) {
+            const int v = tid + 256 * u;
+            const int kr = v / (BN / VEC), nv = (v % (BN / VEC)) * VEC;
+#pragma unroll
+            for (int p = 0; p < 2; p++) *reinterpret_cast<vec_t*>(&sB[buf][p][kr][nv]) = gb[p][u];
+        }
+    };
+
+    constexpr int NACC = sizeof(T) == 4 ? 3 : 2;
+    acc_t acc[NACC][TM][TN];
+#pragma unroll
+    for (int a = 0; a < NACC; a++)
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++) acc[a][i][j] = acc_t{};
+
+    const int nk = K / BK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int ka = kk + (lane >> 4);
+            T ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+            for (int i = 0; i < TM; i++) {
+                ar[i] = sA[buf][0][ka][wm0 + i * 16 + (lane & 15)];
+                ai[i] = sA[buf][1][ka][wm0 + i * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                br[j] = sB[buf][0][ka][wn0 + j * 16 + (lane & 15)];
+                bi[j] = sB[buf][1][ka][wn0 + j * 16 + (lane & 15)];
+            }
+            if constexpr (sizeof(T) == 4) {
+                T as[TM], bs[TN];
+#pragma unroll
+                for (int i = 0; i < TM; i++) as[i] = ar[i] + ai[i];
+#pragma unroll
+                for (int j = 0; j < TN; j++) bs[j] = br[j] + bi[j];
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++) {
+                        acc[0][i][j] = mfma16x16x4(ar[i], br[j], acc[0][i][j]);
+                        acc[1][i][j] = mfma16x16x4(ai[i], bi[j], acc[1][i][j]);
+                        acc[2][i][j] = mfma16x16x4(as[i], bs[j], acc[2][i][j]);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++) {
+                        acc[0][i][j] = mfma16x16x4(ar[i], br[j], acc[0][i][j]);
+                        acc[0][i][j] = mfma16x16x4(ai[i], -bi[j], acc[0][i][j]);
+                        acc[1][i][j] = mfma16x16x4(ar[i], bi[j], acc[1][i][j]);
+                        acc[1][i][j] = mfma16x16x4(ai[i], br[j], acc[1][i][j]);
+                    }
+            }
+        }
+        if (kt + 1 < nk) store(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            const int col = col0 + wn0 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = row0 + wm0 + i * 16 + (sizeof(T) == 4 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r);
+                if (row >= M || col >= N) continue;
+                T vr, vi;
+                if constexpr (sizeof(T) == 4) {
+                    vr = acc[0][i][j][r] - acc[1][i][j][r];
+                    vi = acc[2][i][j][r] - acc[0][i][j][r] - acc[1][i][j][r];
+                } else {
+                    vr = acc[0][i][j][r];
+                    vi = acc[1][i][j][r];
+                }
+                const long o = (long)row * ldc + col;
+                if (mode == 0) {
+                    Cr[o] -= vr;
+                    Ci[o] -= vi;
+                } else if (mode == 1) {
+                    Cr[o] = vr;
+                    Ci[o] = vi;
+                } else {
+                    Cr[o] += vr;
+                    Ci[o] += vi;
+                }
+            }
+        }
+}
+
+// Host launcher.  K must be a multiple of BK (16); N a multiple of the vector
+// width (4 floats / 2 doubles) and every leading dimension a multiple of it.
+template <typename T>
+inline void cgemm(hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br, const T* Bi,
+                  long ldb, T* Cr, T* Ci, long ldc, int mode) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    typedef GemmCfg<T> C;
+    dim3 grid((N + C::BN - 1) / C::BN, (M + C::BM - 1) / C::BM);
+    hipLaunchKernelGGL(k_cgemm<T>, grid, dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+}

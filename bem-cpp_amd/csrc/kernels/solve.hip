@@ -1,0 +1,199 @@
+// solve.hip -- multi-RHS solve with the LU factors and FP64 iterative
+// refinement.  Replaces cuSOLVER Zgetrs (solver.cu:51-84, 125-128).
+//
+// Right-hand sides and solutions are planar FP64, row-major n x ldr (one RHS
+// per column), resident on the device.  A solve with the factors of P Z = L U:
+//   Y = P B;  Y <- L^{-1} Y;  Y <- U^{-1} Y
+// in 256-row blocks: a triangular solve of the diagonal block in LDS
+// (trsm.h) followed by a rank-256 GEMM update of the remaining rows on the
+// matrix cores (gemm.h), for all right-hand sides at once.
+//
+// Mixed precision (factors in FP32): X_0 = 0, R_0 = B, and per sweep
+//   D = solve32(fp32(P R));  X += D;  R = B - Z X   (FP64 GEMM with the FP64 Z)
+// until max_j ||R_j|| / ||B_j|| <= tol.  Norm reductions use a fixed order.
+#include "../bem_internal.h"
+#include "gemm.h"
+#include "trsm.h"
+
+namespace {
+
+constexpr int NB = 256;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_rows(int n, int ncol, const int* __restrict__ perm,
+                                                     const double* __restrict__ Br, const double* __restrict__ Bi,
+                                                     long ldb, T* __restrict__ Yr, T* __restrict__ Yi, long ldy) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * ncol) return;
+    const int i = (int)(idx / ncol), j = (int)(idx % ncol);
+    const long src = (long)perm[i] * ldb + j;
+    Yr[(long)i * ldy + j] = (T)Br[src];
+    Yi[(long)i * ldy + j] = (T)Bi[src];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_accumulate(long count, const T* __restrict__ Dr, const T* __restrict__ Di,
+                                                    double* __restrict__ Xr, double* __restrict__ Xi, int assign) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    if (assign) {
+        Xr[i] = (double)Dr[i];
+        Xi[i] = (double)Di[i];
+    } else {
+        Xr[i] += (double)Dr[i];
+        Xi[i] += (double)Di[i];
+    }
+}
+
+// out[j] = sum_{i < n} |A[i][j]|^2, one block per column, fixed reduction order.
+__global__ __launch_bounds__(256) void k_colnorm2(int n, const double* __restrict__ Ar, const double* __restrict__ Ai,
+                                                  long ld, double* __restrict__ out) {
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double a = Ar[(long)i * ld + j], b = Ai[(long)i * ld + j];
+        s += a * a + b * b;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// Y <- U^{-1} L^{-1} Y for the factors at (Fr, Fi), Y planar n x ncol (ld ldy).
+template <typename T>
+void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, T* Yr, T* Yi, int ncol, long ldy) {
+    const int n = c->npad, nb = n / NB;
+    const long ld = n;
+    hipStream_t s = c->stream;
+    const dim3 tg((ncol + 15) / 16);
+    for (int I = 0; I < nb; I++) {
+        const long d = (long)I * NB * ld + (long)I * NB, y = (long)I * NB * ldy;
+        hipLaunchKernelGGL((k_trsm_lower_unit<T, 16>), tg, dim3(256), 0, s, NB, Fr + d, Fi + d, ld, Yr + y, Yi + y,
+                           ldy, ncol);
+        const int rest = n - (I + 1) * NB;
+        if (rest > 0) {
+            const long a = (long)(I + 1) * NB * ld + (long)I * NB, yc = (long)(I + 1) * NB * ldy;
+            cgemm<T>(s, rest, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr + yc, Yi + yc, ldy, 0);
+        }
+    }
+    for (int I = nb - 1; I >= 0; I--) {
+        const long d = (long)I * NB * ld + (long)I * NB, y = (long)I * NB * ldy;
+        hipLaunchKernelGGL((k_trsm_upper<T, 16>), tg, dim3(256), 0, s, NB, Fr + d, Fi + d, ld, Yr + y, Yi + y, ldy,
+                           ncol);
+        if (I > 0) {
+            const long a = (long)I * NB;  // rows 0 .. I*NB-1, columns of block I
+            cgemm<T>(s, I * NB, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr, Yi, ldy, 0);
+        }
+    }
+}
+
+double max_ratio(const std::vector<double>& r2, const std::vector<double>& b2, int nrhs) {
+    double m = 0.0;
+    for (int j = 0; j < nrhs; j++) {
+        const double v = b2[j] > 0 ? std::sqrt(r2[j] / b2[j]) : std::sqrt(r2[j]);
+        m = std::max(m, v);
+    }
+    return m;
+}
+
+}  // namespace
+
+int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, double* res0, double* res,
+                    double* t_solve_ms, double* t_ir_ms) {
+    const int n = c->npad, ldr = c->ldr, ncol = ldr;
+    const long cnt = (long)n * ldr;
+    hipStream_t s = c->stream;
+    const bool f32 = c->factored == BEM_F32_3M;
+    const bool can_refine = c->has_Z64;
+    // scratch: R (FP64 planar), Y (working precision planar), norms
+    const size_t ysz = f32 ? sizeof(float) : sizeof(double);
+    const size_t bytes = 2 * cnt * sizeof(double) + 2 * cnt * ysz + 2 * (size_t)ldr * sizeof(double) + 256;
+    void* scr = nullptr;
+    if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
+    double* Rr = static_cast<double*>(scr);
+    double* Ri = Rr + cnt;
+    char* yp = reinterpret_cast<char*>(Ri + cnt);
+    double* nb2 = reinterpret_cast<double*>(yp + 2 * cnt * ysz);
+    double* nr2 = nb2 + ldr;
+    hipEvent_t e[4];
+    for (auto& x : e) BEM_HIP(c, hipEventCreate(&x));
+    const dim3 g1((unsigned)((cnt + 255) / 256));
+    const dim3 gr((unsigned)(((long)n * ncol + 255) / 256));
+    float ts = 0.0f, ti = 0.0f;
+    std::vector<double> b2(ldr), r2(ldr);
+
+    auto solve_into_X = [&](const double* Sr, const double* Si, int assign) {
+        if (f32) {
+            float* Yr = reinterpret_cast<float*>(yp);
+            float* Yi = Yr + cnt;
+            hipLaunchKernelGGL(k_gather_rows<float>, gr, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ldr, Yr, Yi,
+                               (long)ldr);
+            lu_apply<float>(c, c->Wr, c->Wi, Yr, Yi, ncol, ldr);
+            hipLaunchKernelGGL(k_accumulate<float>, g1, dim3(256), 0, s, cnt, Yr, Yi, c->Xr, c->Xi, assign);
+        } else {
+            double* Yr = reinterpret_cast<double*>(yp);
+            double* Yi = Yr + cnt;
+            hipLaunchKernelGGL(k_gather_rows<double>, gr, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ldr, Yr,
+                               Yi, (long)ldr);
+            lu_apply<double>(c, c->Vr, c->Vi, Yr, Yi, ncol, ldr);
+            hipLaunchKernelGGL(k_accumulate<double>, g1, dim3(256), 0, s, cnt, Yr, Yi, c->Xr, c->Xi, assign);
+        }
+    };
+    auto residual = [&]() -> int {  // R = B - Z X, returns max relative residual in *res
+        BEM_HIP(c, hipMemcpyAsync(Rr, c->Br, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
+        BEM_HIP(c, hipMemcpyAsync(Ri, c->Bi, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
+        cgemm<double>(s, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0);
+        hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, s, c->n, Rr, Ri, (long)ldr, nr2);
+        BEM_HIP(c, hipMemcpyAsync(r2.data(), nr2, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
+        BEM_HIP(c, hipStreamSynchronize(s));
+        return 0;
+    };
+
+    // plain solve
+    BEM_HIP(c, hipEventRecord(e[0], s));
+    solve_into_X(c->Br, c->Bi, 1);
+    BEM_HIP(c, hipEventRecord(e[1], s));
+    BEM_HIP(c, hipEventSynchronize(e[1]));
+    BEM_HIP(c, hipEventElapsedTime(&ts, e[0], e[1]));
+    *iters = 0;
+    *res0 = *res = 0.0;
+    int rc = 0;
+    if (can_refine && (maxit > 0 || f32)) {
+        BEM_HIP(c, hipEventRecord(e[2], s));
+        hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, s, c->n, c->Br, c->Bi, (long)ldr, nb2);
+        BEM_HIP(c, hipMemcpyAsync(b2.data(), nb2, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (int e2 = residual()) return e2;
+        *res0 = *res = max_ratio(r2, b2, nrhs);
+        int it = 0;
+        float ts_loop = 0.0f;  // triangular solves inside the sweeps
+        while (it < maxit && *res > tol) {
+            float t0 = 0.0f;
+            BEM_HIP(c, hipEventRecord(e[0], s));
+            solve_into_X(Rr, Ri, 0);
+            BEM_HIP(c, hipEventRecord(e[1], s));
+            if (int e2 = residual()) return e2;
+            BEM_HIP(c, hipEventElapsedTime(&t0, e[0], e[1]));
+            ts_loop += t0;
+            *res = max_ratio(r2, b2, nrhs);
+            it++;
+        }
+        *iters = it;
+        BEM_HIP(c, hipEventRecord(e[3], s));
+        BEM_HIP(c, hipEventSynchronize(e[3]));
+        float tall = 0.0f;
+        BEM_HIP(c, hipEventElapsedTime(&tall, e[2], e[3]));
+        ts += ts_loop;          // all triangular solves
+        ti = tall - ts_loop;    // residual GEMMs, norms, updates
+        if (maxit > 0 && *res > tol) rc = BEM_ERR_NOCONV;
+    }
+    BEM_HIP(c, hipGetLastError());
+    for (auto& x : e) (void)hipEventDestroy(x);
+    *t_solve_ms = ts;
+    *t_ir_ms = std::max(0.0f, ti);
+    return rc;
+}

@@ -1,0 +1,174 @@
+/* bem_abi.h -- C ABI of the MI355X dense PMCHWT library (libbem_amd.so).
+ *
+ * The reference (KirillSalnikov/BEM-CPP) drives its dense path through C++
+ * functions with host buffers (pmchwt.h, assembly.h, solver.h, rhs.h,
+ * farfield.h).  This ABI replaces those call sites with plain C entry points:
+ * plain pointers and sizes, interleaved (re, im) doubles for complex numbers
+ * (the memory layout of std::complex<double>), integer return codes.
+ *
+ * State lives on the device inside an opaque context (mesh, RWG basis, Z in
+ * FP64 and FP32, LU factors and pivots, right-hand sides, solutions), so the
+ * pipeline assemble -> factor -> solve -> far field never crosses PCIe except
+ * for the small inputs and outputs.  One context is driven by one host thread;
+ * work is ordered on the context's HIP stream and every call returns after its
+ * work is complete, matching the reference's synchronous semantics.
+ *
+ * Return codes: 0 = success, > 0 = LAPACK-style info from a factorisation
+ * (i+1 for the first zero pivot), < 0 = error (text in bem_last_error).  The
+ * library never exits the process (the reference exits via CUDA_CHECK,
+ * types.h:12-35).
+ */
+#ifndef BEM_ABI_H
+#define BEM_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bem_ctx bem_ctx;
+
+enum {
+    BEM_OK = 0,
+    BEM_ERR_ARG = -1,     /* invalid argument or size */
+    BEM_ERR_STATE = -2,   /* call out of order (e.g. solve before factor) */
+    BEM_ERR_HIP = -3,     /* HIP runtime error */
+    BEM_ERR_NOMEM = -4,   /* device allocation failed */
+    BEM_ERR_NOCONV = -5,  /* iterative refinement did not reach the tolerance */
+    BEM_ERR_MESH = -6     /* mesh has no interior edge / non-finite coordinates */
+};
+
+/* Factorisation precision (bem_factor). */
+enum {
+    BEM_F64 = 0,     /* complex FP64 blocked LU (the reference's Zgetrf semantics) */
+    BEM_F32_3M = 1   /* complex FP32 blocked LU, trailing updates as 3 real MFMA GEMMs */
+};
+
+/* Assembly flags (bem_assemble). */
+enum {
+    BEM_ASM_Z64 = 1u, /* keep the FP64 copy of Z (needed by get_Z, the FP64 LU and refinement) */
+    BEM_ASM_Z32 = 2u  /* build the FP32 copy of Z (needed by the FP32 LU) */
+};
+
+typedef struct {
+    int iterations;       /* refinement sweeps performed (0 = plain solve) */
+    int converged;        /* 1 if the relative residual reached ir_tol */
+    double residual0;     /* max_j ||b_j - Z x_j|| / ||b_j|| after the plain solve (FP64 Z) */
+    double residual;      /* the same after the last sweep */
+    double t_solve_ms;    /* device time of triangular solves */
+    double t_ir_ms;       /* device time of FP64 residuals + corrections */
+} bem_solve_stats;
+
+/* ---- context ----------------------------------------------------------- */
+bem_ctx* bem_create(int device);
+void bem_destroy(bem_ctx* ctx);
+const char* bem_last_error(const bem_ctx* ctx);
+/* Device-time breakdown of the last calls, in ms:
+ * [0] assembly  [1] factorisation  [2] solve  [3] refinement  [4] far field
+ * [5] rhs generation  [6] adjoint.  Returns the number written (<= n). */
+int bem_timings(const bem_ctx* ctx, double* out, int n);
+
+/* ---- host-side generators (no device work) ------------------------------ */
+/* Replace icosphere / hex_prism (mesh.h:21-27, mesh.cpp:11-154): shape 0 is
+ * the icosphere of radius `param`, shape 1 the hexagonal prism of unit
+ * equal-volume radius and aspect ratio `param`.  bem_mesh_counts gives the
+ * vertex and triangle counts; bem_make_mesh fills verts (nv x 3) and tris
+ * (nt x 3) in the reference's order. */
+int bem_mesh_counts(int shape, int refinements, int* nv, int* nt);
+int bem_make_mesh(int shape, int refinements, double param, double* verts, int* tris);
+/* Replaces generate_orientations + sort_orientations_nearest (orient.h:41-47):
+ * writes RT (n x 3 x 3 row-major, lab -> body) and weights w (n); returns n. */
+int bem_orientations(int n_alpha, int n_beta, int n_gamma, int beta_sym, int gamma_sym, int gamma_mirror,
+                     int sort, double* RT, double* w);
+
+/* ---- mesh and basis ---------------------------------------------------- */
+/* Replaces the Mesh/RWG hand-off into assemble_pmchwt (pmchwt.h:12-19):
+ * verts is nv x 3, tris is nt x 3 (0-based).  Builds the RWG basis in the
+ * reference's std::map edge order (rwg.cpp:5-43) and uploads the quadrature
+ * data (tri_quadrature, quadrature.h:14-70; orders 4, 7, 13, others -> 7). */
+int bem_set_mesh(bem_ctx* ctx, const double* verts, int nv, const int* tris, int nt, int quad_order);
+int bem_num_rwg(const bem_ctx* ctx);
+/* RWG arrays as build_rwg returns them (rwg.h:7-16); any pointer may be NULL. */
+int bem_get_rwg(const bem_ctx* ctx, int* tri_p, int* tri_m, double* free_p, double* free_m, double* length,
+                double* area_p, double* area_m);
+
+/* ---- assembly ---------------------------------------------------------- */
+/* Replaces assemble_pmchwt (pmchwt.h:12-19, pmchwt.cu:6-56) and the two
+ * assemble_L_K_cuda calls it makes (assembly.h:11-13): builds
+ * Z = [[ee L_e + ei L_i, -(K_e+K_i)], [K_e+K_i, L_e/ee + L_i/ei]] on the device. */
+int bem_assemble(bem_ctx* ctx, double kext_re, double kext_im, double kint_re, double kint_im, double eta_ext,
+                 double eta_int, unsigned flags);
+/* Copies Z (2N x 2N, FP64) to the host, row-major (row_major=1, the reference
+ * layout of pmchwt.cu:43-49) or column-major. */
+int bem_get_Z(bem_ctx* ctx, double* Z, int row_major);
+/* Replaces assemble_L_K_cuda (assembly.h:11-13): L and K (N x N row-major) for
+ * one wavenumber.  Runs the same kernels with k_ext = k_int = k, eta = 1. */
+int bem_assemble_LK(bem_ctx* ctx, double k_re, double k_im, double* L, double* K);
+
+/* ---- dense LU ---------------------------------------------------------- */
+/* Replaces lu_factorize_cuda (solver.h:11, solver.cu:7-48): factors the
+ * assembled Z in the requested precision.  Returns info (> 0 singular). */
+int bem_factor(bem_ctx* ctx, int precision);
+/* Loads a caller matrix (n x n, interleaved complex, row-major or column-major)
+ * as Z, replacing an assembled one; used by the lu_* drop-ins. */
+int bem_set_Z(bem_ctx* ctx, const double* Z, int n, int row_major);
+/* Copies the 1-based LAPACK-style pivot vector of the last factorisation
+ * (ipiv of lu_factorize_cuda, solver.h:11) into ipiv[n]. */
+int bem_get_pivots(const bem_ctx* ctx, int* ipiv);
+/* The factors of the last FP64 factorisation, P Z = L U with unit L, as one
+ * n x n interleaved complex matrix (L below, U on and above the diagonal) in
+ * row-major order: what lu_factorize_cuda leaves in Z (solver.h:11). */
+int bem_get_lu(bem_ctx* ctx, double* LU);
+/* Loads factors and 1-based pivots in the bem_get_lu/bem_get_pivots layout as
+ * the current FP64 factorisation: the (Z, ipiv) input of lu_solve_cuda
+ * (solver.h:16-17). */
+int bem_set_lu(bem_ctx* ctx, const double* LU, const int* ipiv, int n);
+/* Replaces lu_solve_cuda / lu_solve_full (solver.h:16-22, solver.cu:51-142):
+ * solves Z X = B.  B and X are n x nrhs column-major (one RHS per contiguous
+ * column, solver.h:13-15; main.cpp:739-751); X may alias B.  B == NULL solves
+ * the right-hand sides left on the device by bem_rhs_planewave; X == NULL
+ * keeps the solution on the device only.  With the FP32 factorisation the
+ * solution is refined against the FP64 Z until max_j ||b_j - Z x_j||/||b_j||
+ * <= ir_tol or ir_max sweeps (ir_max = 0: no refinement).  stats may be NULL. */
+int bem_solve(bem_ctx* ctx, const double* B, int nrhs, double* X, double ir_tol, int ir_max, bem_solve_stats* stats);
+
+/* ---- excitation -------------------------------------------------------- */
+/* Replaces compute_rhs_planewave (rhs.h:11-16, rhs.cpp:6-64), batched: RHS j
+ * is the plane wave with direction k_hat[3j..3j+2] and polarisation
+ * E0[3j..3j+2].  Keeps B (2N x nrhs) on the device; B_out (nullable) receives
+ * it column-major. */
+int bem_rhs_planewave(bem_ctx* ctx, double kext_re, double kext_im, double eta_ext, const double* k_hat,
+                      const double* E0, int nrhs, double* B_out);
+
+/* ---- far field and Mueller ---------------------------------------------- */
+/* Replaces compute_farfield_batch_cuda (farfield.h:41-48, farfield.cu:267-346):
+ * coeffs_J/M are n_calls x N (call c at offset c*N), r_hats n_orient x ndir x 3
+ * with call c using orientation c/2, Fv_out n_calls x ndir x 3 complex. */
+int bem_farfield(bem_ctx* ctx, const double* coeffs_J, const double* coeffs_M, const double* r_hats, double kext_re,
+                 double kext_im, double eta_ext, int n_calls, int n_orient, int ndir, double* Fv_out);
+/* The dense orientation loop of main.cpp:758-846 on the device solution of
+ * bem_solve (column 2o = parallel, 2o+1 = perpendicular polarisation of
+ * orientation o): far field at theta[t] in the lab xz-plane rotated by RT
+ * (n_orient x 3 x 3 row-major), amplitudes S1..S4, Mueller matrix (Bohren &
+ * Huffman 3.16, farfield.cu:464-508), optional gamma-mirror zeroing, divided
+ * by |k|^2 and summed with weights w in orientation order.  M_avg is 4 x 4 x
+ * ntheta ([(i*4+j)*ntheta + t], output.cpp layout).  M_orient (nullable)
+ * receives the unweighted per-orientation matrices (n_orient x 16 x ntheta). */
+int bem_orient_mueller(bem_ctx* ctx, double kext_re, double kext_im, double eta_ext, const double* RT,
+                       const double* w, int n_orient, const double* theta, int ntheta, int gamma_mirror,
+                       double* M_avg, double* M_orient);
+
+/* ---- adjoint ----------------------------------------------------------- */
+/* No counterpart in the reference (it has no derivatives).  Gradient of a
+ * real loss of the averaged Mueller matrix of the last bem_orient_mueller
+ * call, given dL/dM_avg (4 x 4 x ntheta), with respect to (Re m, Im m, ka)
+ * for k_ext = ka, k_int = ka m, eta_ext = 1, eta_int = 1/|m|
+ * (main.cpp:192-196).  Uses one adjoint solve with the existing
+ * factorisation (Z^T = S Z S, S = diag(I, -I)) and a matrix-free contraction
+ * of dZ/dp.  grad receives 3 doubles. */
+int bem_mueller_grad(bem_ctx* ctx, const double* dL_dM, double* grad);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEM_ABI_H */

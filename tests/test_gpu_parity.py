@@ -1,0 +1,240 @@
+"""GPU parity tests: the HIP library (through the C ABI) against the FP64
+oracle and the reference twin's golden vectors, on inputs the oracle finishes
+in seconds.  Tolerances are stated per test."""
+import numpy as np
+import pytest
+
+from conftest import golden, python_to_sorted
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def bem():
+    import bem_amd
+    return bem_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(bem):
+    c = bem.Context(0)
+    yield c
+    c.close()
+
+
+def _perm2(keys):
+    order = python_to_sorted(keys)
+    N = len(order)
+    return np.concatenate([order, order + N])
+
+
+# Z: far triangle pairs use FP32 sin/cos/exp of an FP64-reduced phase (relative
+# error ~1e-7 per point pair), near pairs are FP64.  Entry-wise bound:
+# |dZ| <= 2e-6 |Z| + 1e-7 max|Z|.
+def _z_close(Z, Zr, rtol=2e-6, atol=1e-7):
+    scale = np.abs(Zr).max()
+    bad = np.abs(Z - Zr) > rtol * np.abs(Zr) + atol * scale
+    return not bad.any(), np.abs(Z - Zr).max() / scale
+
+
+@pytest.mark.parametrize("shape,ref,param,ka,m", [
+    ("sphere", 0, 1.0, 1.0, 1.3116 + 0j),
+    ("sphere", 1, 1.0, 2.0, 1.33 + 0.02j),
+    ("sphere", 2, 1.0, 1.0, 1.3116 + 0j),
+    ("hex", 1, 1.4286, 3.0, 1.3116 + 0.01j),
+    ("hex", 0, 0.7, 1.5, 1.3116 + 0j),
+])
+def test_Z_vs_oracle(oracle, ctx, shape, ref, param, ka, m):
+    p = oracle.Problem.from_shape(shape, ref, param)
+    ctx.set_mesh(p.verts, p.tris, 7)
+    args = (complex(ka), ka * m, 1.0, 1.0 / abs(m))
+    ctx.assemble(*args)
+    Z = ctx.get_Z()
+    Zo = p.assemble_pmchwt(*args)
+    ok, err = _z_close(Z, Zo)
+    print(f"{shape} ref={ref}: max|dZ|/max|Z| = {err:.2e}")
+    assert ok, err
+    S = np.concatenate([np.ones(p.N), -np.ones(p.N)])
+    assert np.abs(Z.T - S[:, None] * Z * S[None, :]).max() < 1e-12 * np.abs(Z).max()
+
+
+@pytest.mark.parametrize("quad", [4, 13])
+def test_Z_other_quadratures(oracle, ctx, quad):
+    p = oracle.Problem.from_shape("sphere", 1, quad_order=quad)
+    ctx.set_mesh(p.verts, p.tris, quad)
+    args = (1.5 + 0j, 1.5 * (1.4 + 0.01j), 1.0, 1.0 / abs(1.4 + 0.01j))
+    ctx.assemble(*args)
+    ok, err = _z_close(ctx.get_Z(), p.assemble_pmchwt(*args))
+    assert ok, err
+
+
+@pytest.mark.parametrize("tag", ["ico0_real", "hex0"])
+def test_Z_vs_reference_golden(ctx, tag):
+    g = golden(f"{tag}_Z.npz")
+    ctx.set_mesh(g["verts"], g["tris"], 7)
+    ka, m = float(g["ka"]), complex(g["m"])
+    ctx.assemble(complex(ka), ka * m, 1.0, 1.0 / abs(m))
+    P = _perm2(g["keys"])
+    ok, err = _z_close(ctx.get_Z(), g["Z"][np.ix_(P, P)])
+    assert ok, err
+
+
+def test_LK_vs_oracle(oracle, ctx):
+    p = oracle.Problem.from_shape("sphere", 1)
+    ctx.set_mesh(p.verts, p.tris, 7)
+    k = 2.5 + 0.03j
+    L, K = ctx.assemble_LK(k)
+    Lo, Ko = p.assemble_LK(k)
+    assert _z_close(L, Lo)[0] and _z_close(K, Ko, atol=1e-6)[0]
+
+
+# ---------------------------------------------------------------- LU
+@pytest.mark.parametrize("n,nrhs", [(40, 1), (300, 5), (700, 17), (1024, 3)])
+def test_lu_fp64_vs_numpy(ctx, n, nrhs):
+    rs = np.random.RandomState(42)
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n)
+    B = rs.randn(n, nrhs) + 1j * rs.randn(n, nrhs)
+    ctx.set_Z(Z)
+    assert ctx.factor("fp64") == 0
+    X, st = ctx.solve(B, ir_max=0)
+    Xr = np.linalg.solve(Z, B)
+    cond = np.linalg.cond(Z)
+    assert np.abs(X - Xr).max() <= 1e-13 * cond * np.abs(Xr).max()
+    # normwise backward error; a stable pivoted LU gives O(n eps)
+    be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
+    assert be < 2e-15
+
+
+@pytest.mark.parametrize("n,nrhs", [(300, 4), (1100, 32)])
+def test_lu_fp32_refined_vs_numpy(ctx, n, nrhs):
+    rs = np.random.RandomState(42)
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n) + 4 * np.sqrt(n) * np.eye(n)
+    B = rs.randn(n, nrhs) + 1j * rs.randn(n, nrhs)
+    ctx.set_Z(Z)
+    assert ctx.factor("fp32") == 0
+    X, st = ctx.solve(B, ir_tol=1e-12, ir_max=20)
+    assert st["converged"] == 1 and st["residual"] <= 1e-12
+    assert st["residual0"] > st["residual"]
+    Xr = np.linalg.solve(Z, B)
+    assert np.abs(X - Xr).max() <= 1e-10 * np.abs(Xr).max()
+
+
+def test_lu_singular_info(ctx):
+    rs = np.random.RandomState(3)
+    Z = rs.randn(64, 64) + 0j
+    Z[:, 10] = 0.0
+    ctx.set_Z(Z)
+    info = ctx.factor("fp64")
+    assert info > 0
+
+
+def test_lu_get_set_roundtrip(ctx):
+    rs = np.random.RandomState(5)
+    n = 90
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n)
+    b = rs.randn(n) + 1j * rs.randn(n)
+    ctx.set_Z(Z)
+    assert ctx.factor("fp64") == 0
+    LU, ip = ctx.get_lu(n), ctx.get_pivots(n)
+    P = np.eye(n)[:, :]
+    perm = np.arange(n)
+    for i in range(n):
+        j = ip[i] - 1
+        perm[[i, j]] = perm[[j, i]]
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    assert np.abs(L @ U - Z[perm]).max() < 1e-12 * np.abs(Z).max()
+    ctx.set_lu(LU, ip)
+    x, _ = ctx.solve(b, ir_max=0)
+    assert np.abs(Z @ x[:, 0] - b).max() < 1e-11 * np.abs(b).max() * np.linalg.cond(Z)
+
+
+# ---------------------------------------------------------------- RHS / far field
+def test_rhs_vs_oracle(oracle, ctx, bem):
+    p = oracle.Problem.from_shape("hex", 1, 1.4286)
+    ctx.set_mesh(p.verts, p.tris, 7)
+    RT, w = bem.orientations(3, 2, 1)
+    kh, e0 = bem.planewave_orient_rhs(RT)
+    k = 2.0 + 0j
+    B = ctx.rhs_planewave(k, 1.0, kh, e0, return_B=True)
+    Bo = np.stack([p.rhs_planewave(k, 1.0, e0[j], kh[j]) for j in range(len(kh))], axis=1)
+    assert np.abs(B - Bo).max() < 1e-13 * np.abs(Bo).max()
+
+
+def test_farfield_vs_oracle(oracle, ctx):
+    p = oracle.Problem.from_shape("sphere", 1)
+    ctx.set_mesh(p.verts, p.tris, 7)
+    rs = np.random.RandomState(42)
+    N = p.N
+    cJ = rs.randn(3, N) + 1j * rs.randn(3, N)
+    cM = rs.randn(3, N) + 1j * rs.randn(3, N)
+    rh = rs.randn(2, 7, 3)
+    rh /= np.linalg.norm(rh, axis=2)[:, :, None]
+    k = 1.7 + 0j
+    Fv = ctx.farfield(cJ, cM, rh, k, 1.0, n_calls=3, n_orient=2)
+    for c in range(3):
+        Fo = p.farfield_vec(cJ[c], cM[c], k, 1.0, rh[c // 2])
+        assert np.abs(Fv[c] - Fo).max() < 1e-12 * np.abs(Fo).max()
+
+
+# ---------------------------------------------------------------- pipelines
+MUELLER_GATE = 1e-4  # max_ij,theta |dM_ij| / M11(theta)
+
+
+def _mueller_err(M, Mr):
+    return (np.abs(M - Mr) / Mr[0, 0][None, None, :]).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_c1_single_vs_reference(ctx, bem, precision):
+    g = golden("c1_mueller.npz")
+    v, t = bem.make_mesh("sphere", 2)
+    ctx.set_mesh(v, t, 7)
+    m = 1.3116 + 0j
+    ctx.assemble(1.0 + 0j, 1.0 * m, 1.0, 1.0 / abs(m))
+    RT, w = bem.lab_single_orientation()
+    kh, e0 = bem.planewave_orient_rhs(RT)
+    ctx.rhs_planewave(1.0, 1.0, kh, e0)
+    assert ctx.factor(precision) == 0
+    X, st = ctx.solve(None, ir_tol=1e-10, ir_max=10, return_X=False, nrhs=2)
+    M = ctx.orient_mueller(1.0, 1.0, RT, w, g["theta"])
+    err = _mueller_err(M, g["M"])
+    print(f"C1 {precision}: Mueller error {err:.2e}, stats {st}")
+    assert err < MUELLER_GATE
+
+
+@pytest.mark.parametrize("fix,shape,ref,param,ka,orient", [
+    ("ico1_orient.npz", "sphere", 1, 1.0, 1.0, (2, 2, 1)),
+    ("hex0_orient.npz", "hex", 0, 1.4286, 1.5, (3, 2, 2)),
+])
+def test_orientation_average_vs_reference(ctx, bem, fix, shape, ref, param, ka, orient):
+    g = golden(fix)
+    v, t = bem.make_mesh(shape, ref, param)
+    ctx.set_mesh(v, t, 7)
+    m = 1.3116 + 0j
+    ctx.assemble(complex(ka), ka * m, 1.0, 1.0 / abs(m))
+    RT, w = bem.orientations(*orient)
+    kh, e0 = bem.planewave_orient_rhs(RT)
+    ctx.rhs_planewave(ka, 1.0, kh, e0)
+    assert ctx.factor("fp32") == 0
+    ctx.solve(None, ir_tol=1e-10, ir_max=10, return_X=False, nrhs=len(kh))
+    M = ctx.orient_mueller(ka, 1.0, RT, w, g["theta"])
+    assert _mueller_err(M, g["M"]) < MUELLER_GATE
+
+
+def test_orient_vs_oracle_lossy_hex(oracle, ctx, bem):
+    p = oracle.Problem.from_shape("hex", 1, 1.4286)
+    m = 1.3116 + 0.01j
+    ka = 2.0
+    info, Mo, Xo = p.dense_orient(ka, m, 3, 3, 1, ntheta=37)
+    ctx.set_mesh(p.verts, p.tris, 7)
+    ctx.assemble(complex(ka), ka * m, 1.0, 1.0 / abs(m))
+    RT, w = bem.orientations(3, 3, 1)
+    kh, e0 = bem.planewave_orient_rhs(RT)
+    ctx.rhs_planewave(ka, 1.0, kh, e0)
+    assert ctx.factor("fp32") == 0
+    X, st = ctx.solve(None, ir_tol=1e-10, ir_max=10, nrhs=len(kh))
+    th = np.linspace(0, np.pi, 37)
+    M = ctx.orient_mueller(ka, 1.0, RT, w, th)
+    assert _mueller_err(M, Mo) < MUELLER_GATE
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-5
