@@ -69,7 +69,7 @@ def lib():
     vp, i, d, u = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint
     sig = {
         "bem_create": ([i], vp), "bem_destroy": ([vp], None), "bem_last_error": ([vp], ctypes.c_char_p),
-        "bem_timings": ([vp, _dp, i], i),
+        "bem_timings": ([vp, _dp, i], i), "bem_kernel_stats": ([vp, i, _dp, i], i),
         "bem_mesh_counts": ([i, i, _ip, _ip], i), "bem_make_mesh": ([i, i, d, _dp, _ip], i),
         "bem_orientations": ([i, i, i, i, i, i, i, _dp, _dp], i),
         "bem_set_mesh": ([vp, _dp, i, _ip, i, i], i), "bem_num_rwg": ([vp], i),
@@ -271,6 +271,19 @@ class Context:
         d = _cd(dL_dM)
         self._check(self._L.bem_mueller_grad(self._h, _p(d), _p(g)), "mueller_grad")
         return g
+
+    KERNEL_CLASSES = ("gemm_fp32_3m", "gemm_fp64", "asm_far", "asm_near", "asm_gather", "tslu", "laswp", "panel",
+                      "trsm", "farfield", "rhs", "other")
+
+    def kernel_stats(self, mode=None):
+        """mode 'on' / 'off' reset the counters; None returns
+        {class: (launches, ms, flops, bytes)}."""
+        if mode in ("on", "off"):
+            self._L.bem_kernel_stats(self._h, 1 if mode == "on" else 0, None, 0)
+            return None
+        out = np.zeros(4 * len(self.KERNEL_CLASSES))
+        n = self._L.bem_kernel_stats(self._h, 2, _p(out), len(self.KERNEL_CLASSES))
+        return {self.KERNEL_CLASSES[i]: tuple(out[4 * i:4 * i + 4]) for i in range(n)}
 
     def timings(self):
         out = np.zeros(8)

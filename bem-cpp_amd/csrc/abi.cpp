@@ -30,7 +30,33 @@ int ctx_scratch(bem_ctx* c, size_t bytes, void** out) {
     return 0;
 }
 
+void kt_resolve(bem_ctx* c) {
+    KernelTimer& k = c->kt;
+    if (k.pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (KernelTimer::Rec& r : k.pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            k.ms[r.cls] += ms;
+            k.flops[r.cls] += r.flops;
+            k.bytes[r.cls] += r.bytes;
+            k.launches[r.cls] += 1;
+        }
+        k.pool.push_back(r.a);
+        k.pool.push_back(r.b);
+    }
+    k.pending.clear();
+}
+
 namespace {
+
+// Resolves kernel timings on scope exit of an ABI call.
+struct KtScope {
+    bem_ctx* c;
+    ~KtScope() {
+        if (c) kt_resolve(c);
+    }
+};
 
 template <class T>
 void dfree(T*& p) {
@@ -184,6 +210,29 @@ int bem_timings(const bem_ctx* c, double* out, int n) {
     if (!c || !out || n < 0) return BEM_ERR_ARG;
     const int m = std::min(n, 8);
     for (int i = 0; i < m; i++) out[i] = c->t_ms[i];
+    return m;
+}
+
+int bem_kernel_stats(bem_ctx* c, int mode, double* out, int n) {
+    if (!c) return BEM_ERR_ARG;
+    KernelTimer& k = c->kt;
+    kt_resolve(c);
+    if (mode == 0 || mode == 1) {
+        k.on = mode == 1;
+        for (int i = 0; i < KS_COUNT; i++) {
+            k.ms[i] = k.flops[i] = k.bytes[i] = 0.0;
+            k.launches[i] = 0;
+        }
+        return KS_COUNT;
+    }
+    if (mode != 2 || !out) return BEM_ERR_ARG;
+    const int m = std::min(n, (int)KS_COUNT);
+    for (int i = 0; i < m; i++) {
+        out[4 * i] = (double)k.launches[i];
+        out[4 * i + 1] = k.ms[i];
+        out[4 * i + 2] = k.flops[i];
+        out[4 * i + 3] = k.bytes[i];
+    }
     return m;
 }
 
@@ -350,6 +399,7 @@ int bem_get_rwg(const bem_ctx* c, int* tri_p, int* tri_m, double* free_p, double
 
 int bem_assemble(bem_ctx* c, double ker, double kei, double kir, double kii, double ee, double ei, unsigned flags) {
     if (!c) return BEM_ERR_ARG;
+    KtScope kts{c};
     if (!c->has_mesh) return bem_fail(c, BEM_ERR_STATE, "assemble: no mesh");
     if (!(std::isfinite(ker) && std::isfinite(kei) && std::isfinite(kir) && std::isfinite(kii)) ||
         std::hypot(ker, kei) == 0.0 || std::hypot(kir, kii) == 0.0)
@@ -445,6 +495,7 @@ int bem_set_Z(bem_ctx* c, const double* Z, int n, int row_major) {
 
 int bem_factor(bem_ctx* c, int precision) {
     if (!c) return BEM_ERR_ARG;
+    KtScope kts{c};
     if (precision != BEM_F64 && precision != BEM_F32_3M) return bem_fail(c, BEM_ERR_ARG, "factor: bad precision");
     if (!c->has_Z64 && !(precision == BEM_F32_3M && c->has_Z32))
         return bem_fail(c, BEM_ERR_STATE, "factor: no matrix (assemble or set_Z first)");
@@ -536,6 +587,7 @@ int bem_set_lu(bem_ctx* c, const double* LU, const int* ipiv, int n) {
 
 int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, int ir_max, bem_solve_stats* st) {
     if (!c) return BEM_ERR_ARG;
+    KtScope kts{c};
     if (c->factored < 0) return bem_fail(c, BEM_ERR_STATE, "solve: no factorisation");
     if (nrhs < 1 && B) return bem_fail(c, BEM_ERR_ARG, "solve: nrhs must be positive");
     if (ir_max < 0 || !(ir_tol >= 0.0)) return bem_fail(c, BEM_ERR_ARG, "solve: bad refinement parameters");
@@ -590,6 +642,7 @@ int bem_rhs_planewave(bem_ctx* c, double kr, double ki, double eta, const double
                       double* B_out) {
     if (!c || !khat || !E0 || nrhs < 1) return bem_fail(c, BEM_ERR_ARG, "rhs: bad arguments");
     if (!c->has_mesh) return bem_fail(c, BEM_ERR_STATE, "rhs: no mesh");
+    KtScope kts{c};
     if (c->n != 2 * c->rwg.N) return bem_fail(c, BEM_ERR_STATE, "rhs: system size does not match the mesh");
     (void)hipSetDevice(c->device);
     if (int rc = ctx_ensure_rhs(c, nrhs)) return rc;
@@ -617,6 +670,7 @@ int bem_farfield(bem_ctx* c, const double* cJ, const double* cM, const double* r
         return bem_fail(c, BEM_ERR_ARG, "farfield: bad arguments");
     if ((n_calls + 1) / 2 > n_orient) return bem_fail(c, BEM_ERR_ARG, "farfield: n_calls needs ceil(n_calls/2) orientations");
     if (!c->has_mesh) return bem_fail(c, BEM_ERR_STATE, "farfield: no mesh");
+    KtScope kts{c};
     (void)hipSetDevice(c->device);
     const int N = c->rwg.N;
     // coefficient planes (n_calls x N each for J and M), directions, output
@@ -649,6 +703,7 @@ int bem_orient_mueller(bem_ctx* c, double kr, double ki, double eta, const doubl
     if (!c || !RT || !w || !theta || !M_avg || n_orient < 1 || ntheta < 1)
         return bem_fail(c, BEM_ERR_ARG, "orient_mueller: bad arguments");
     if (!c->has_X) return bem_fail(c, BEM_ERR_STATE, "orient_mueller: no solution on the device");
+    KtScope kts{c};
     if (c->nrhs < 2 * n_orient) return bem_fail(c, BEM_ERR_ARG, "orient_mueller: needs 2 solutions per orientation");
     if (c->n != 2 * c->rwg.N) return bem_fail(c, BEM_ERR_STATE, "orient_mueller: solution does not match the mesh");
     (void)hipSetDevice(c->device);

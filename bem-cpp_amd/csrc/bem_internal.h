@@ -32,8 +32,40 @@ struct AsmParams {
     double eta[2];        // eta_ext, eta_int
 };
 
+// Per-kernel-class device timing (bem_kernel_stats): HIP events around each
+// launch on the context stream, resolved when the ABI call returns.
+enum KernelClass {
+    KS_GEMM32 = 0,   // complex GEMM, FP32 3M on MFMA
+    KS_GEMM64,       // complex GEMM, FP64 4M on MFMA
+    KS_TRIPAIR_FAR,  // assembly, non-touching triangle pairs
+    KS_TRIPAIR_NEAR, // assembly, touching pairs + Graglia
+    KS_GATHER,       // assembly, Z from half-pair records
+    KS_TSLU,         // pivot tournament
+    KS_LASWP,        // row interchanges
+    KS_PANEL,        // panel factor (diagonal block, L21, U12 strip)
+    KS_TRSM,         // triangular solves in LDS
+    KS_FARFIELD,     // far field partial sums
+    KS_RHS,          // plane-wave right-hand sides
+    KS_OTHER,
+    KS_COUNT
+};
+
+struct KernelTimer {
+    bool on = false;
+    struct Rec {
+        hipEvent_t a, b;
+        int cls;
+        double flops, bytes;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[KS_COUNT] = {}, flops[KS_COUNT] = {}, bytes[KS_COUNT] = {};
+    long launches[KS_COUNT] = {};
+};
+
 struct bem_ctx {
     int device = 0;
+    KernelTimer kt;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::string err;
@@ -111,3 +143,32 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad);
 // Small helpers used by several launchers
 int ctx_ensure_rhs(bem_ctx* c, int nrhs);
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Kernel timing helpers (no-ops unless enabled).  kt_mark returns a record
+// index to close with kt_close after the launch.
+inline int kt_mark(bem_ctx* c, int cls) {
+    KernelTimer& k = c->kt;
+    if (!k.on) return -1;
+    KernelTimer::Rec r{};
+    for (hipEvent_t* e : {&r.a, &r.b}) {
+        if (!k.pool.empty()) {
+            *e = k.pool.back();
+            k.pool.pop_back();
+        } else if (hipEventCreate(e) != hipSuccess) {
+            return -1;
+        }
+    }
+    r.cls = cls;
+    (void)hipEventRecord(r.a, c->stream);
+    k.pending.push_back(r);
+    return (int)k.pending.size() - 1;
+}
+inline void kt_close(bem_ctx* c, int idx, double flops, double bytes) {
+    if (idx < 0) return;
+    KernelTimer::Rec& r = c->kt.pending[idx];
+    r.flops = flops;
+    r.bytes = bytes;
+    (void)hipEventRecord(r.b, c->stream);
+}
+// Waits for the recorded launches and folds them into the totals.
+void kt_resolve(bem_ctx* c);

@@ -555,18 +555,25 @@ int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi
     hipLaunchKernelGGL(k_graglia<NQ>, dim3((T * NQ + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv, dverts,
                        dqw, d.pv);
     const int nb = (T + 15) / 16;
+    const double npair = (double)T * (T + 1) / 2;
+    int kt = kt_mark(c, KS_TRIPAIR_FAR);
     if (lossy)
         hipLaunchKernelGGL((k_tripair_far<NQ, true>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv, d.hc,
                            d.hp, p, c->H);
     else
         hipLaunchKernelGGL((k_tripair_far<NQ, false>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv,
                            d.hc, d.hp, p, c->H);
+    kt_close(c, kt, npair * NQ * NQ, npair * kRec * 8.0);  // unit: triangle point pairs
+    kt = kt_mark(c, KS_TRIPAIR_NEAR);
     if (d.n_near > 0)
         hipLaunchKernelGGL(k_tripair_near<NQ>, dim3((d.n_near + 127) / 128), dim3(128), 0, c->stream, T, d.n_near,
                            d.near, d.qp, d.hc, d.hp, d.pv, p, c->H);
+    kt_close(c, kt, (double)d.n_near * NQ * NQ, (double)d.n_near * kRec * 8.0);
     const int N = d.N;
+    kt = kt_mark(c, KS_GATHER);
     hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256, N), dim3(256), 0, c->stream, N, T, c->npad, d.rwg_half, c->H,
                        Zr, Zi, Wr, Wi);
+    kt_close(c, kt, 0.0, (double)N * N * (4.0 * 48.0 + (Zr ? 64.0 : 0.0) + (Wr ? 32.0 : 0.0)));
     const int n = 2 * N;
     if (c->npad > n) {
         if (Zr)

@@ -307,8 +307,10 @@ int farfield_core(bem_ctx* c, const double* Jr, const double* Ji, const double* 
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_current, dim3((unsigned)((P * n_calls + 255) / 256)), dim3(256), 0, s, d.T, d.Nq, n_calls,
                        d.qp, d.hc, d.hp, d.hrwg, Jr, Ji, Mr, Mi, sc, sn, cur);
+    const int kt = kt_mark(c, KS_FARFIELD);
     hipLaunchKernelGGL(k_ff_partial, dim3(n_orient, nchunk), dim3(256), 0, s, P, n_calls, ndir, d.qp, cur, drh, kr,
                        ki, part);
+    kt_close(c, kt, (double)n_orient * ndir * P * (8.0 * 12.0), (double)n_orient * P * (4.0 + 24.0) * 8.0);
     hipLaunchKernelGGL(k_ff_final, dim3((n_orient * ndir + 255) / 256), dim3(256), 0, s, n_orient, n_calls, ndir,
                        nchunk, part, drh, kr, ki, eta, dFv);
     return 0;
@@ -340,9 +342,11 @@ int rhs_launch(bem_ctx* c, double kr, double ki, double eta, const double* khat,
     BEM_HIP(c, hipMemcpyAsync(s, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     const DevMesh& d = c->dm;
     const long cnt = (long)c->npad * c->ldr;
+    const int kt = kt_mark(c, KS_RHS);
     hipLaunchKernelGGL(k_rhs, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, d.N, d.Nq, c->npad,
                        c->ldr, nrhs, d.qp, d.hc, d.hp, d.rwg_half, static_cast<const double*>(s), kr, ki, c->Br,
                        c->Bi);
+    kt_close(c, kt, (double)d.N * nrhs * 2 * d.Nq, (double)cnt * 16.0);
     BEM_HIP(c, hipGetLastError());
     BEM_HIP(c, hipStreamSynchronize(c->stream));
     return 0;

@@ -299,6 +299,7 @@ int factor(bem_ctx* c, T* Ar, T* Ai, int* dinfo, int* cid, T* cvr, T* cvi) {
             T *vr = cvr, *vi = cvi;
             int* co = cid + (size_t)n / PB * PB;  // second half of the candidate buffers
             T *wr = cvr + (size_t)n * PB, *wi = cvi + (size_t)n * PB;
+            int kt = kt_mark(c, KS_TSLU);
             hipLaunchKernelGGL(k_tslu<T>, dim3(nsets), dim3(256), 0, s, 0, Ar, Ai, ld, j0, n, 0, nullptr, nullptr,
                                nullptr, ci, vr, vi, nsets == 1 ? 1 : 0, c->ipiv);
             while (nsets > 1) {
@@ -311,14 +312,19 @@ int factor(bem_ctx* c, T* Ar, T* Ai, int* dinfo, int* cid, T* cvr, T* cvi) {
                 std::swap(vi, wi);
                 nsets = nout;
             }
+            kt_close(c, kt, 0.0, 2.0 * sizeof(T) * (double)m * PB);
+            kt = kt_mark(c, KS_LASWP);
             hipLaunchKernelGGL(k_laswp<T>, dim3((n + 255) / 256), dim3(256), 0, s, Ar, Ai, ld, n, j0, PB, c->ipiv);
+            kt_close(c, kt, 0.0, 8.0 * sizeof(T) * (double)n * PB);
             const int nright = k0 + NB - (j0 + PB);
             const int nblk = 1 + (m - PB + 255) / 256;
+            kt = kt_mark(c, KS_PANEL);
             hipLaunchKernelGGL(k_panel<T>, dim3(nblk), dim3(256), 0, s, Ar, Ai, ld, j0, m, nright, dinfo);
+            kt_close(c, kt, 8.0 * (double)m * PB * PB / 2, 4.0 * sizeof(T) * (double)m * PB);
             if (nright > 0) {
                 const long o21 = (long)(j0 + PB) * ld + j0, o12 = (long)j0 * ld + j0 + PB;
                 const long o22 = (long)(j0 + PB) * ld + j0 + PB;
-                cgemm<T>(s, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22,
+                cgemm_ctx<T>(c, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22,
                          ld, 0);
             }
         }
@@ -326,9 +332,11 @@ int factor(bem_ctx* c, T* Ar, T* Ai, int* dinfo, int* cid, T* cvr, T* cvi) {
         if (nr > 0) {
             const long o11 = (long)k0 * ld + k0, o12 = (long)k0 * ld + k0 + NB;
             const long o21 = (long)(k0 + NB) * ld + k0, o22 = (long)(k0 + NB) * ld + k0 + NB;
+            const int kt = kt_mark(c, KS_TRSM);
             hipLaunchKernelGGL((k_trsm_lower_unit<T, 16>), dim3((nr + 15) / 16), dim3(256), 0, s, NB, Ar + o11,
                                Ai + o11, ld, Ar + o12, Ai + o12, ld, nr);
-            cgemm<T>(s, nr, nr, NB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22, ld, 0);
+            kt_close(c, kt, 4.0 * (double)NB * NB * nr, 4.0 * sizeof(T) * (double)NB * nr);
+            cgemm_ctx<T>(c, nr, nr, NB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22, ld, 0);
         }
     }
     return 0;

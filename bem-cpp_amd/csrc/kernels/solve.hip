@@ -73,21 +73,25 @@ void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, T* Yr, T* Yi, int ncol, long
     const dim3 tg((ncol + 15) / 16);
     for (int I = 0; I < nb; I++) {
         const long d = (long)I * NB * ld + (long)I * NB, y = (long)I * NB * ldy;
+        int kt = kt_mark(c, KS_TRSM);
         hipLaunchKernelGGL((k_trsm_lower_unit<T, 16>), tg, dim3(256), 0, s, NB, Fr + d, Fi + d, ld, Yr + y, Yi + y,
                            ldy, ncol);
+        kt_close(c, kt, 4.0 * (double)NB * NB * ncol, 4.0 * sizeof(T) * (double)NB * ncol);
         const int rest = n - (I + 1) * NB;
         if (rest > 0) {
             const long a = (long)(I + 1) * NB * ld + (long)I * NB, yc = (long)(I + 1) * NB * ldy;
-            cgemm<T>(s, rest, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr + yc, Yi + yc, ldy, 0);
+            cgemm_ctx<T>(c, rest, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr + yc, Yi + yc, ldy, 0);
         }
     }
     for (int I = nb - 1; I >= 0; I--) {
         const long d = (long)I * NB * ld + (long)I * NB, y = (long)I * NB * ldy;
+        int kt = kt_mark(c, KS_TRSM);
         hipLaunchKernelGGL((k_trsm_upper<T, 16>), tg, dim3(256), 0, s, NB, Fr + d, Fi + d, ld, Yr + y, Yi + y, ldy,
                            ncol);
+        kt_close(c, kt, 4.0 * (double)NB * NB * ncol, 4.0 * sizeof(T) * (double)NB * ncol);
         if (I > 0) {
             const long a = (long)I * NB;  // rows 0 .. I*NB-1, columns of block I
-            cgemm<T>(s, I * NB, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr, Yi, ldy, 0);
+            cgemm_ctx<T>(c, I * NB, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr, Yi, ldy, 0);
         }
     }
 }
@@ -147,7 +151,7 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
     auto residual = [&]() -> int {  // R = B - Z X, returns max relative residual in *res
         BEM_HIP(c, hipMemcpyAsync(Rr, c->Br, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
         BEM_HIP(c, hipMemcpyAsync(Ri, c->Bi, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
-        cgemm<double>(s, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0);
+        cgemm_ctx<double>(c, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0);
         hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, s, c->n, Rr, Ri, (long)ldr, nr2);
         BEM_HIP(c, hipMemcpyAsync(r2.data(), nr2, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
         BEM_HIP(c, hipStreamSynchronize(s));

@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the dense PMCHWT path on one MI355X.
+
+One step = the whole dense pipeline on device-resident inputs (mesh and basis
+uploaded before timing): assemble Z (FP64 + FP32 copies) -> 128 plane-wave
+right-hand sides -> FP32 LU on the matrix cores -> multi-RHS solve with FP64
+iterative refinement to 1e-10 -> far field, amplitude and Mueller matrices,
+orientation average over 64 orientations.
+
+Default workload (the one the metric is quoted on): hexagonal prism, aspect
+ratio 0.7, 4 refinements (T = 6144 triangles, N = 9216 RWG, 2N = 18432),
+ka = 10, m = 1.3116, 7-point quadrature, Gauss-Legendre 8 x 8 x 1 orientations
+x 2 polarisations = 128 RHS, 181 scattering angles.
+
+After the timed steps the same system is factored in FP64 on the GPU (the
+FP64-LU comparison and the FP64 reference for the Mueller error), and the FP64
+CPU oracle is timed on a bounded sample of the same workload on the host cores.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this benchmark measures
+one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "bem-cpp_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+WORKLOADS = {
+    # name: (shape, ref, aspect/radius, ka, m, orient grid)
+    "c1": ("sphere", 2, 1.0, 1.0, 1.3116 + 0j, (1, 1, 1)),
+    "c2": ("sphere", 3, 1.0, 5.0, 1.3116 + 0.01j, (1, 1, 1)),
+    "c3": ("hex", 3, 1.4286, 5.0, 1.3116 + 0j, (8, 8, 1)),
+    "c5": ("hex", 4, 0.7, 10.0, 1.3116 + 0j, (8, 8, 1)),
+}
+METRIC = "Z entries/s; s/orient FP32-LU+FP64 IR vs FP64 LU @ Mueller err ≤1e-4"
+PEAK_TFLOPS = {"gemm_fp32_3m": 157.3, "gemm_fp64": 78.6}  # MI355X dense MFMA, AMD specification
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def mueller_err(M, Mref):
+    return float((np.abs(M - Mref) / np.abs(Mref[0, 0])[None, None, :]).max())
+
+
+def cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N_full):
+    """FP64 oracle on the host cores, on a bounded sample, scaled to s/orient of
+    the full workload: assembly (point pairs ~ N^2) from the mesh two
+    refinements coarser, factor + 128-RHS solve (~ n^3) from a random system of
+    n = 3072, far field (~ points per orientation) from two orientations of the
+    coarser mesh."""
+    from oracle import oracle
+
+    threads = oracle.num_threads()
+    ref_s = max(ref - 2, 0)
+    p = oracle.Problem.from_shape(shape, ref_s, param)
+    scale_pts = (N_full / p.N)
+    t0 = time.perf_counter()
+    p.assemble_pmchwt(complex(ka), ka * m, 1.0, 1.0 / abs(m))
+    t_asm = (time.perf_counter() - t0) * scale_pts ** 2
+    n_s = 3072
+    rs = np.random.RandomState(42)
+    Z = rs.randn(n_s, n_s) + 1j * rs.randn(n_s, n_s)
+    B = rs.randn(n_s, 2 * n_orient) + 1j * rs.randn(n_s, 2 * n_orient)
+    t0 = time.perf_counter()
+    oracle.lu_solve(Z, B)
+    t_lu = (time.perf_counter() - t0) * (2.0 * N_full / n_s) ** 3
+    info, Mo, X = p.dense_orient(ka, m, 2, 1, 1, ntheta=ntheta)
+    t_ff_per_orient = p.last_timings()["farfield_s"] / 2 * scale_pts
+    total = t_asm + t_lu + t_ff_per_orient * n_orient
+    return {
+        "value": total / n_orient, "unit": "s/orient", "cores": threads, "kind": "port",
+        "sample": (f"FP64 oracle (oracle/bem_oracle.cpp, OpenMP {threads} threads): assembly of {shape} ref {ref_s} "
+                   f"(N={p.N}) x (N/{p.N})^2, LU + {2 * n_orient}-RHS solve of a random n={n_s} system x "
+                   f"(2N/{n_s})^3, far field of 2 orientations on ref {ref_s} x points ratio; extrapolated to "
+                   f"N={N_full}: assembly {t_asm:.0f} s, LU+solve {t_lu:.0f} s, far field "
+                   f"{t_ff_per_orient * n_orient:.1f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
+    ap.add_argument("--ir-tol", type=float, default=1e-10)
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by orientation, "
+              "see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import bem_amd
+
+    shape, ref, param, ka, m, orient = WORKLOADS[args.workload]
+    ntheta = 181
+    verts, tris = bem_amd.make_mesh(shape, ref, param)
+    RT, w = bem_amd.orientations(*orient) if orient != (1, 1, 1) else bem_amd.lab_single_orientation()
+    kh, e0 = bem_amd.planewave_orient_rhs(RT)
+    n_orient, nrhs = len(w), len(kh)
+    theta = np.pi * np.arange(ntheta) / (ntheta - 1)
+    k_ext, k_int, eta_int = complex(ka), ka * m, 1.0 / abs(m)
+
+    ctx = bem_amd.Context(0)
+    N = ctx.set_mesh(verts, tris, 7)
+    n = 2 * N
+    log(f"workload {args.workload}: {shape} ref {ref} T={len(tris)} N={N} 2N={n} ka={ka} m={m} "
+        f"orient={n_orient} rhs={nrhs}")
+
+    def step():
+        ctx.assemble(k_ext, k_int, 1.0, eta_int, z64=True, z32=True)
+        ta = ctx.timings()["assembly_ms"]
+        ctx.rhs_planewave(k_ext, 1.0, kh, e0)
+        tr = ctx.timings()["rhs_ms"]
+        info = ctx.factor("fp32")
+        if info:
+            raise RuntimeError(f"FP32 factorisation info {info}")
+        _, st = ctx.solve(None, ir_tol=args.ir_tol, ir_max=20, return_X=False, nrhs=nrhs, allow_noconv=True)
+        t = ctx.timings()
+        M = ctx.orient_mueller(k_ext, 1.0, RT, w, theta)
+        t = ctx.timings()
+        return M, st, dict(assembly=ta, rhs=tr, factor=t["factor_ms"], solve=t["solve_ms"], refine=t["refine_ms"],
+                           farfield=t["farfield_ms"])
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        log(f"warmup {i}: {time.perf_counter() - t0:.3f} s")
+    ctx.kernel_stats("on")
+    phases = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        M32, st, ph = step()
+        phases.append(ph)
+        log(f"step {i}: {time.perf_counter() - ts:.3f} s  {ph}  IR sweeps {st['iterations']} "
+            f"residual {st['residual']:.2e}")
+    elapsed = time.perf_counter() - t0
+    kstats = ctx.kernel_stats()
+    ctx.kernel_stats("off")
+    ms_per_step = elapsed / args.steps * 1e3
+    ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+
+    # FP64 LU on the same GPU: the comparison and the FP64 reference for the Mueller error
+    fp64 = {}
+    if not args.skip_fp64:
+        ctx.factor("fp64")
+        f64_factor = ctx.timings()["factor_ms"]
+        X, st64 = ctx.solve(None, ir_tol=0.0, ir_max=0, return_X=False, nrhs=nrhs)
+        f64_solve = ctx.timings()["solve_ms"]
+        M64 = ctx.orient_mueller(k_ext, 1.0, RT, w, theta)
+        f64_ff = ctx.timings()["farfield_ms"]
+        fp64 = dict(factor_ms=f64_factor, solve_ms=f64_solve, farfield_ms=f64_ff,
+                    s_per_orient=(ph["rhs"] + f64_factor + f64_solve + f64_ff) / n_orient / 1e3,
+                    mueller_err_fp32ir_vs_fp64=mueller_err(M32, M64))
+        log(f"fp64: {fp64}")
+
+    # roofline of the dominant kernel class (live HIP-event times over the timed steps)
+    dom = max(kstats, key=lambda k: kstats[k][1])
+    launches, kms, kflops, kbytes = kstats[dom]
+    if dom in PEAK_TFLOPS:
+        ach = kflops / (kms * 1e-3) / 1e12
+        roof = {"kernel": "k_cgemm<float> (3M complex GEMM, LU trailing updates + solves)" if dom == "gemm_fp32_3m"
+                else "k_cgemm<double>", "bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[dom],
+                "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom], "traffic": None,
+                "launches": int(launches), "avg_launch_ms": kms / max(launches, 1)}
+    else:
+        ach = kbytes / (kms * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None, "launches": int(launches),
+                "avg_launch_ms": kms / max(launches, 1)}
+
+    step_s = ms_per_step / 1e3
+    s_orient_excl_asm = (ph["rhs"] + ph["factor"] + ph["solve"] + ph["refine"] + ph["farfield"]) / n_orient / 1e3
+    out = {
+        "metric": METRIC,
+        "value": step_s / n_orient,
+        "unit": "s/orient",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 LU/solve on MFMA (3M) + fp64 refinement, assembly sums, far field",
+        "data": "synthetic (deterministic mesh and plane waves)",
+        "config": {
+            "workload": (f"{shape} {'AR ' + str(param) if shape == 'hex' else 'r=1'} ref {ref}: T={len(tris)}, "
+                         f"N={N}, 2N={n}; ka={ka}, m={m.real}{m.imag:+}i; 7-point quadrature; "
+                         f"{n_orient} orientations x 2 polarisations = {nrhs} RHS; {ntheta} angles"),
+            "n": n, "nrhs": nrhs, "n_orient": n_orient, "ntheta": ntheta,
+        },
+        "z_entries_per_s": n * n / (ph["assembly"] * 1e-3),
+        "s_per_orient_fp32ir_excl_assembly": s_orient_excl_asm,
+        "s_per_orient_fp64_excl_assembly": fp64.get("s_per_orient"),
+        "speedup_fp32ir_vs_fp64": (fp64["s_per_orient"] / s_orient_excl_asm) if fp64 else None,
+        "mueller_err_vs_fp64": fp64.get("mueller_err_fp32ir_vs_fp64"),
+        "ir_sweeps": st["iterations"], "ir_residual": st["residual"],
+        "phase_ms": ph,
+        "fp64_phase_ms": {k: fp64[k] for k in ("factor_ms", "solve_ms", "farfield_ms")} if fp64 else None,
+        "kernels": {k: {"launches": int(v[0]), "ms": v[1], "tflops": (v[2] / (v[1] * 1e-3) / 1e12) if v[1] else 0.0}
+                    for k, v in kstats.items() if v[0] > 0},
+        "roofline": roof,
+    }
+    ctx.close()
+    if not args.skip_cpu:
+        out["cpu_baseline"] = cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -66,6 +66,15 @@ const char* bem_last_error(const bem_ctx* ctx);
  * [0] assembly  [1] factorisation  [2] solve  [3] refinement  [4] far field
  * [5] rhs generation  [6] adjoint.  Returns the number written (<= n). */
 int bem_timings(const bem_ctx* ctx, double* out, int n);
+/* Per-kernel-class device time, measured with HIP events on the context
+ * stream around every launch (no counterpart in the reference; used for the
+ * roofline figures).  mode 1: reset and enable, 0: reset and disable,
+ * 2: read.  For class i, out[4i..4i+3] = launches, total ms, counted flops,
+ * counted bytes.  Classes: 0 GEMM FP32-3M, 1 GEMM FP64, 2 assembly far
+ * pairs, 3 assembly near pairs, 4 assembly gather, 5 pivot tournament,
+ * 6 row swaps, 7 panel, 8 triangular solves, 9 far field, 10 RHS, 11 other.
+ * Returns the number of classes. */
+int bem_kernel_stats(bem_ctx* ctx, int mode, double* out, int n);
 
 /* ---- host-side generators (no device work) ------------------------------ */
 /* Replace icosphere / hex_prism (mesh.h:21-27, mesh.cpp:11-154): shape 0 is
