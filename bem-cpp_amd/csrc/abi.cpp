@@ -98,6 +98,10 @@ void free_matrices(bem_ctx* c) {
     dfree(c->Vi);
     dfree(c->ipiv);
     dfree(c->perm);
+    if (c->inv32) (void)hipFree(c->inv32);
+    if (c->inv64) (void)hipFree(c->inv64);
+    c->inv32 = c->inv64 = nullptr;
+    c->inv32_bytes = c->inv64_bytes = 0;
     c->has_Z64 = c->has_Z32 = false;
     c->factored = -1;
 }
@@ -580,6 +584,7 @@ int bem_set_lu(bem_ctx* c, const double* LU, const int* ipiv, int n) {
     BEM_HIP(c, hipMemcpy(c->Vi, im.data(), nn * sizeof(double), hipMemcpyHostToDevice));
     BEM_HIP(c, hipMemcpy(c->ipiv, ip.data(), ld * sizeof(int), hipMemcpyHostToDevice));
     BEM_HIP(c, hipMemcpy(c->perm, perm.data(), ld * sizeof(int), hipMemcpyHostToDevice));
+    if (int rc = lu_inverses_launch(c)) return rc;
     c->factored = BEM_F64;
     c->has_Z64 = false;  // no system matrix: solves run without refinement
     return 0;

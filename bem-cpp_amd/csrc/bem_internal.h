@@ -86,6 +86,10 @@ struct bem_ctx {
     float* Wi = nullptr;
     double* Vr = nullptr;  // FP64 LU factors
     double* Vi = nullptr;
+    void* inv32 = nullptr;  // inverses of the 256 x 256 diagonal blocks of L and U (FP32 factors)
+    size_t inv32_bytes = 0;
+    void* inv64 = nullptr;  // the same for the FP64 factors
+    size_t inv64_bytes = 0;
     int* ipiv = nullptr;   // [npad] LAPACK-style swaps (0-based row swapped with row i)
     int* perm = nullptr;   // [npad] row permutation: row i of PZ is row perm[i] of Z
     int factored = -1;     // -1 none, BEM_F64, BEM_F32_3M
@@ -130,6 +134,7 @@ int ctx_scratch(bem_ctx* c, size_t bytes, void** out);
 // Launchers (kernels/*.hip).  All enqueue on c->stream and return 0 or < 0.
 int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32);
 int lu_factor_launch(bem_ctx* c, int precision);
+int lu_inverses_launch(bem_ctx* c);  // FP64 diagonal-block inverses of factors in Vr/Vi
 int lu_solve_launch(bem_ctx* c, int nrhs, double ir_tol, int ir_max, int* iters, double* res0, double* res,
                     double* t_solve_ms, double* t_ir_ms);
 int rhs_launch(bem_ctx* c, double kr, double ki, double eta, const double* khat, const double* E0, int nrhs);

@@ -7,30 +7,31 @@
 // The pivots differ from cuSOLVER's; the solution does not.
 //
 // Right-looking two-level blocking on the padded matrix (npad = 256 k):
-//   outer panel of NB = 256 columns, factored in 8 inner steps of B = 32:
+//   outer panel of NB = 256 columns, factored in 8 inner steps of PB = 32:
 //     1. pivot selection for the 32 columns over all remaining rows by a
 //        tournament (communication-avoiding "TSLU": Grigori, Demmel & Xiang,
-//        SIAM J. Matrix Anal. Appl. 32 (2011)): 256-row chunks each choose 32
-//        candidate rows by partial pivoting in LDS, candidates are merged
-//        8 sets at a time until one set remains; its pivot order becomes
-//        LAPACK-style swaps ipiv[j0 .. j0+31]
+//        SIAM J. Matrix Anal. Appl. 32 (2011)), in one launch: 256-row chunks
+//        each choose 32 candidate rows by partial pivoting in LDS; the last
+//        block of every group of 8 to finish (integer arrival counter) merges
+//        the group's candidates, up to the root, whose pivot order becomes
+//        LAPACK-style swaps ipiv[j0 .. j0+31].  No block ever waits.
 //     2. the swaps are applied to whole rows (all columns)
 //     3. the 32 x 32 diagonal block is factored without further pivoting,
 //        L21 = A21 U11^{-1} row by row, U12 of the rest of the outer panel
-//     4. rank-32 update of the rest of the outer panel
-//   then U12 = L11^{-1} A12 (256 x n') and the rank-256 trailing update
-//   A22 -= L21 U12 on the matrix cores (gemm.h; 3M form in FP32).
-// Each step is a separate launch on the context stream: no grid-wide barrier
-// and no atomics, so every wave always runs to completion.
+//     4. rank-32 update of the rest of the outer panel on the matrix cores
+//   then L11^{-1} of the 256 x 256 diagonal block (trsm.h), U12 = L11^{-1} A12
+//   and the rank-256 trailing update A22 -= L21 U12 as matrix-core GEMMs
+//   (gemm.h; 3M form in FP32).  The inverses of all L and U diagonal blocks
+//   are kept for the GEMM-based multi-RHS solves.
 #include "../bem_internal.h"
 #include "gemm.h"
 #include "trsm.h"
 
 namespace {
 
-constexpr int NB = 256;  // outer panel width
-constexpr int PB = 32;   // inner panel width (pivot block)
-constexpr int CH = 256;  // rows per tournament block
+constexpr int NB = 256;     // outer panel width = solve block size
+constexpr int PB = 32;      // inner panel width (pivot block)
+constexpr int CH = 256;     // rows per tournament block
 constexpr int G = CH / PB;  // candidate sets merged per block
 
 template <typename T>
@@ -66,104 +67,138 @@ __device__ __forceinline__ void argmax256(T val, int idx, T* sv, int* si, T& bes
     __syncthreads();
 }
 
-// ---------------------------------------------------------------------------
-// Tournament step.  Loads up to CH candidate rows (32 columns each) either
-// straight from the matrix (level 0: rows row0 + blockIdx.x*CH ..) or from
-// the previous level's candidate sets (G sets per block), runs partial
-// pivoting in LDS, and writes the chosen 32 rows (ids and original values) as
-// candidate set blockIdx.x.  With `final`, also turns the pivot order into
-// LAPACK-style swaps ipiv[j0 + c] (0-based row indices).
-// ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_tslu(int level, const T* __restrict__ Ar, const T* __restrict__ Ai,
-                                              long ld, int j0, int row_end, int nsets_in,
-                                              const int* __restrict__ cid_in, const T* __restrict__ cvr_in,
-                                              const T* __restrict__ cvi_in, int* __restrict__ cid_out,
-                                              T* __restrict__ cvr_out, T* __restrict__ cvi_out, int final_level,
-                                              int* __restrict__ ipiv) {
-    __shared__ T vr[CH][PB + 1], vi[CH][PB + 1];
-    __shared__ int id[CH], src[CH];
-    __shared__ T sv[4];
-    __shared__ int si[4];
+struct TsluShared {
+    T vr[CH][PB + 1], vi[CH][PB + 1];
+    int id[CH], src[CH];
+    T sv[4];
+    int si[4];
+    int last;
+};
+
+// Partial pivoting on the nrows rows in LDS: rows 0..31 end up as the chosen
+// rows in pivot order (id/src permuted along).
+template <typename T>
+__device__ void gepp_select(TsluShared<T>& S, int nrows) {
     const int t = threadIdx.x;
-    int nrows;
-    if (level == 0) {
-        const int r0 = j0 + blockIdx.x * CH;
-        nrows = min(CH, row_end - r0);
-        if (t < nrows) {
-            const long o = (long)(r0 + t) * ld + j0;
-            for (int c = 0; c < PB; c++) {
-                vr[t][c] = Ar[o + c];
-                vi[t][c] = Ai[o + c];
-            }
-            id[t] = r0 + t;
-            src[t] = r0 + t;
-        }
-    } else {
-        const int s0 = blockIdx.x * G;
-        const int ns = min(G, nsets_in - s0);
-        nrows = ns * PB;
-        if (t < nrows) {
-            const long o = ((long)s0 * PB + t) * PB;
-            for (int c = 0; c < PB; c++) {
-                vr[t][c] = cvr_in[o + c];
-                vi[t][c] = cvi_in[o + c];
-            }
-            id[t] = cid_in[s0 * PB + t];
-            src[t] = s0 * PB + t;
-        }
-    }
-    __syncthreads();
-    const int nsel = min(PB, nrows);
-    for (int jc = 0; jc < nsel; jc++) {
-        const T v = (t >= jc && t < nrows) ? mag2(vr[t][jc], vi[t][jc]) : T(-1);
+    for (int jc = 0; jc < PB; jc++) {
+        const T v = (t >= jc && t < nrows) ? mag2(S.vr[t][jc], S.vi[t][jc]) : T(-1);
         T best;
         int p;
-        argmax256(v, t, sv, si, best, p);
-        if (p != jc && t < PB) {  // swap LDS rows jc and p
-            T a = vr[jc][t]; vr[jc][t] = vr[p][t]; vr[p][t] = a;
-            a = vi[jc][t]; vi[jc][t] = vi[p][t]; vi[p][t] = a;
+        argmax256(v, t, S.sv, S.si, best, p);
+        if (p != jc && t < PB) {
+            T a = S.vr[jc][t];
+            S.vr[jc][t] = S.vr[p][t];
+            S.vr[p][t] = a;
+            a = S.vi[jc][t];
+            S.vi[jc][t] = S.vi[p][t];
+            S.vi[p][t] = a;
         }
         if (p != jc && t == 0) {
-            int x = id[jc]; id[jc] = id[p]; id[p] = x;
-            x = src[jc]; src[jc] = src[p]; src[p] = x;
+            int x = S.id[jc];
+            S.id[jc] = S.id[p];
+            S.id[p] = x;
+            x = S.src[jc];
+            S.src[jc] = S.src[p];
+            S.src[p] = x;
         }
         __syncthreads();
         if (best > T(0) && t > jc && t < nrows) {
             T lr, li;
-            cdiv(vr[t][jc], vi[t][jc], vr[jc][jc], vi[jc][jc], lr, li);
+            cdiv(S.vr[t][jc], S.vi[t][jc], S.vr[jc][jc], S.vi[jc][jc], lr, li);
             for (int c = jc + 1; c < PB; c++) {
-                const T ur = vr[jc][c], ui = vi[jc][c];
-                vr[t][c] -= lr * ur - li * ui;
-                vi[t][c] -= lr * ui + li * ur;
+                const T ur = S.vr[jc][c], ui = S.vi[jc][c];
+                S.vr[t][c] -= lr * ur - li * ui;
+                S.vi[t][c] -= lr * ui + li * ur;
             }
         }
         __syncthreads();
     }
-    // chosen rows 0..nsel-1: ids and original values
-    if (t < nsel) {
-        cid_out[blockIdx.x * PB + t] = id[t];
-        const long o = ((long)blockIdx.x * PB + t) * PB;
-        if (level == 0) {
-            const long g = (long)src[t] * ld + j0;
-            for (int c = 0; c < PB; c++) {
-                cvr_out[o + c] = Ar[g + c];
-                cvi_out[o + c] = Ai[g + c];
-            }
-        } else {
-            const long g = (long)src[t] * PB;
-            for (int c = 0; c < PB; c++) {
-                cvr_out[o + c] = cvr_in[g + c];
-                cvi_out[o + c] = cvi_in[g + c];
-            }
-        }
+}
+
+// ---------------------------------------------------------------------------
+// Pivot tournament for columns j0..j0+31 over rows j0..n-1, one launch.
+// Candidate sets of level L live at set index off[L] + i; set = 32 ids +
+// 32 x 32 original values per plane.  cnt: one arrival counter per group per
+// level (zero on entry; reset by the merging block).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_tslu_tree(const T* __restrict__ Ar, const T* __restrict__ Ai, long ld,
+                                                   int j0, int n, int* __restrict__ cid, T* __restrict__ cvr,
+                                                   T* __restrict__ cvi, int* __restrict__ cnt, int* __restrict__ ipiv) {
+    __shared__ TsluShared<T> S;
+    const int t = threadIdx.x;
+    // level 0: rows j0 + b*CH .. (coalesced: 32 consecutive columns of a row)
+    const int r0 = j0 + blockIdx.x * CH;
+    int nrows = min(CH, n - r0);
+    for (int e = t; e < nrows * PB; e += 256) {
+        const int r = e / PB, c = e % PB;
+        const long o = (long)(r0 + r) * ld + j0 + c;
+        S.vr[r][c] = Ar[o];
+        S.vi[r][c] = Ai[o];
     }
-    if (final_level && t == 0) {
-        // Pivot order -> sequential swaps: row j0+c is exchanged with the
-        // current position of the c-th chosen row.  A chosen row that sits
-        // at the position being vacated moves to the other side of the swap.
+    if (t < nrows) {
+        S.id[t] = r0 + t;
+        S.src[t] = r0 + t;
+    }
+    __syncthreads();
+    gepp_select(S, nrows);
+    // write candidate set = original rows of the chosen ids
+    int level_sets = (n - j0 + CH - 1) / CH;  // sets at this level
+    int set = blockIdx.x, off = 0, cnt_off = 0;
+    for (int e = t; e < PB * PB; e += 256) {
+        const int r = e / PB, c = e % PB;
+        const long g = (long)S.src[r] * ld + j0 + c;
+        cvr[((long)(off + set) * PB + r) * PB + c] = Ar[g];
+        cvi[((long)(off + set) * PB + r) * PB + c] = Ai[g];
+    }
+    if (t < PB) cid[(off + set) * PB + t] = S.id[t];
+    while (level_sets > 1) {
+        // arrive at the group counter; the last arrival merges the group
+        const int grp = set / G;
+        const int gsize = min(G, level_sets - grp * G);
+        __threadfence();
+        __syncthreads();
+        if (t == 0) {
+            const int prev = atomicAdd(&cnt[cnt_off + grp], 1);
+            S.last = prev == gsize - 1;
+            if (S.last) cnt[cnt_off + grp] = 0;  // ready for the next launch
+        }
+        __syncthreads();
+        if (!S.last) return;
+        __threadfence();
+        // load the group's candidate sets (original values)
+        const int s0 = off + grp * G;
+        nrows = gsize * PB;
+        for (int e = t; e < nrows * PB; e += 256) {
+            const int r = e / PB, c = e % PB;
+            S.vr[r][c] = cvr[((long)s0 * PB + r) * PB + c];
+            S.vi[r][c] = cvi[((long)s0 * PB + r) * PB + c];
+        }
+        if (t < nrows) {
+            S.id[t] = cid[s0 * PB + t];
+            S.src[t] = s0 * PB + t;
+        }
+        __syncthreads();
+        gepp_select(S, nrows);
+        const int noff = off + level_sets;
+        cnt_off += (level_sets + G - 1) / G;
+        set = grp;
+        off = noff;
+        level_sets = (level_sets + G - 1) / G;
+        for (int e = t; e < PB * PB; e += 256) {
+            const int r = e / PB, c = e % PB;
+            const long g = (long)S.src[r] * PB + c;
+            cvr[((long)(off + set) * PB + r) * PB + c] = cvr[g];
+            cvi[((long)(off + set) * PB + r) * PB + c] = cvi[g];
+        }
+        if (t < PB) cid[(off + set) * PB + t] = S.id[t];
+    }
+    // root: pivot order -> sequential swaps; a chosen row sitting at the
+    // position being vacated moves to the other side of the swap
+    if (t == 0) {
         int pos[PB];
-        for (int c = 0; c < PB; c++) pos[c] = c < nsel ? id[c] : j0 + c;
+        for (int c = 0; c < PB; c++) pos[c] = S.id[c];
         for (int c = 0; c < PB; c++) {
             const int x = j0 + c, y = pos[c];
             ipiv[x] = y;
@@ -196,30 +231,39 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict
 // Panel step after the swaps.  Every block factors the 32 x 32 diagonal block
 // (rows/cols j0..j0+31) in LDS without pivoting.  Block 0 writes it back,
 // computes U12 = L11^{-1} A12 for columns j0+32 .. j0+32+nright and records a
-// zero pivot in *info.  Blocks >= 1 compute L21 = A21 U11^{-1} for 256 rows each.
+// zero pivot in *info.  Blocks >= 1 compute L21 = A21 U11^{-1} for 256 rows
+// each, staged through LDS with coalesced loads and stores.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict__ Ai, long ld, int j0, int nrows,
                                                int nright, int* __restrict__ info) {
     __shared__ T ur[PB][PB + 1], ui[PB][PB + 1];
+    __shared__ T lr_s[256][PB + 1], li_s[256][PB + 1];
     const int t = threadIdx.x;
     for (int e = t; e < PB * PB; e += 256) {
         const int r = e / PB, c = e % PB;
         ur[r][c] = Ar[(long)(j0 + r) * ld + j0 + c];
         ui[r][c] = Ai[(long)(j0 + r) * ld + j0 + c];
     }
+    const int rb = j0 + PB + (blockIdx.x - 1) * 256;  // first L21 row of this block
+    const int nr = blockIdx.x == 0 ? 0 : min(256, j0 + nrows - rb);
+    for (int e = t; e < nr * PB; e += 256) {
+        const int r = e / PB, c = e % PB;
+        lr_s[r][c] = Ar[(long)(rb + r) * ld + j0 + c];
+        li_s[r][c] = Ai[(long)(rb + r) * ld + j0 + c];
+    }
     __syncthreads();
     for (int k = 0; k < PB; k++) {
         const T pr = ur[k][k], pi = ui[k][k];
         const bool zero = pr == T(0) && pi == T(0);
         if (!zero && t > k && t < PB) {
-            T lr, li;
-            cdiv(ur[t][k], ui[t][k], pr, pi, lr, li);
-            ur[t][k] = lr;
-            ui[t][k] = li;
+            T a, b;
+            cdiv(ur[t][k], ui[t][k], pr, pi, a, b);
+            ur[t][k] = a;
+            ui[t][k] = b;
             for (int c = k + 1; c < PB; c++) {
-                const T a = ur[k][c], b = ui[k][c];
-                ur[t][c] -= lr * a - li * b;
-                ui[t][c] -= lr * b + li * a;
+                const T x = ur[k][c], y = ui[k][c];
+                ur[t][c] -= a * x - b * y;
+                ui[t][c] -= a * y + b * x;
             }
         }
         __syncthreads();
@@ -237,81 +281,79 @@ __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict
                     break;
                 }
         }
-        // U12 of the rest of the outer panel: unit-lower forward substitution per column
+        // U12 of the rest of the outer panel: unit-lower forward substitution,
+        // one column per thread, the column kept in this thread's row of lr_s
+        // (block 0 stages no L21 rows)
         for (int c = t; c < nright; c += 256) {
             const long base = (long)j0 * ld + j0 + PB + c;
-            T xr[PB], xi[PB];
             for (int r = 0; r < PB; r++) {
-                T sr = Ar[base + (long)r * ld], si = Ai[base + (long)r * ld];
-                for (int l = 0; l < r; l++) {
-                    sr -= ur[r][l] * xr[l] - ui[r][l] * xi[l];
-                    si -= ur[r][l] * xi[l] + ui[r][l] * xr[l];
+                lr_s[t][r] = Ar[base + (long)r * ld];
+                li_s[t][r] = Ai[base + (long)r * ld];
+            }
+            for (int l = 0; l < PB - 1; l++) {
+                const T xr = lr_s[t][l], xi = li_s[t][l];
+                for (int r = l + 1; r < PB; r++) {
+                    const T a = ur[r][l], b = ui[r][l];
+                    lr_s[t][r] -= a * xr - b * xi;
+                    li_s[t][r] -= a * xi + b * xr;
                 }
-                xr[r] = sr;
-                xi[r] = si;
             }
             for (int r = 0; r < PB; r++) {
-                Ar[base + (long)r * ld] = xr[r];
-                Ai[base + (long)r * ld] = xi[r];
+                Ar[base + (long)r * ld] = lr_s[t][r];
+                Ai[base + (long)r * ld] = li_s[t][r];
             }
         }
         return;
     }
     // L21 rows: x U11 = a  ->  x_c = (a_c - sum_{l<c} x_l U[l][c]) / U[c][c]
-    const int row = j0 + PB + (blockIdx.x - 1) * 256 + t;
-    if (row >= j0 + nrows) return;
-    const long o = (long)row * ld + j0;
-    T xr[PB], xi[PB];
-    for (int c = 0; c < PB; c++) {
-        T sr = Ar[o + c], si = Ai[o + c];
-        for (int l = 0; l < c; l++) {
-            sr -= xr[l] * ur[l][c] - xi[l] * ui[l][c];
-            si -= xr[l] * ui[l][c] + xi[l] * ur[l][c];
+    if (t < nr) {
+        for (int c = 0; c < PB; c++) {
+            T sr = lr_s[t][c], si = li_s[t][c];
+            for (int l = 0; l < c; l++) {
+                const T xr = lr_s[t][l], xi = li_s[t][l];
+                sr -= xr * ur[l][c] - xi * ui[l][c];
+                si -= xr * ui[l][c] + xi * ur[l][c];
+            }
+            T qr = sr, qi = si;
+            if (!(ur[c][c] == T(0) && ui[c][c] == T(0))) cdiv(sr, si, ur[c][c], ui[c][c], qr, qi);
+            lr_s[t][c] = qr;
+            li_s[t][c] = qi;
         }
-        T qr = sr, qi = si;
-        if (!(ur[c][c] == T(0) && ui[c][c] == T(0))) cdiv(sr, si, ur[c][c], ui[c][c], qr, qi);
-        xr[c] = qr;
-        xi[c] = qi;
     }
-    for (int c = 0; c < PB; c++) {
-        Ar[o + c] = xr[c];
-        Ai[o + c] = xi[c];
+    __syncthreads();
+    for (int e = t; e < nr * PB; e += 256) {
+        const int r = e / PB, c = e % PB;
+        Ar[(long)(rb + r) * ld + j0 + c] = lr_s[r][c];
+        Ai[(long)(rb + r) * ld + j0 + c] = li_s[r][c];
     }
 }
 
-}  // namespace
-
-namespace {
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_rows(int rows, int cols, const T* __restrict__ Sr,
+                                                   const T* __restrict__ Si, long lds, T* __restrict__ Dr,
+                                                   T* __restrict__ Di, long ldd) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)rows * cols) return;
+    const int r = (int)(idx / cols), c = (int)(idx % cols);
+    Dr[(long)r * ldd + c] = Sr[(long)r * lds + c];
+    Di[(long)r * ldd + c] = Si[(long)r * lds + c];
+}
 
 // Blocked factorisation of the padded npad x npad matrix at (Ar, Ai).
+// work: candidate sets (ids, values), arrival counters, a 256 x n U12 buffer.
 template <typename T>
-int factor(bem_ctx* c, T* Ar, T* Ai, int* dinfo, int* cid, T* cvr, T* cvi) {
+void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, int* dinfo, int* cid, T* cvr, T* cvi, int* cnt,
+            T* ubr, T* ubi) {
     const int n = c->npad;
     const long ld = n;
     hipStream_t s = c->stream;
     for (int k0 = 0; k0 < n; k0 += NB) {
         for (int j0 = k0; j0 < k0 + NB; j0 += PB) {
             const int m = n - j0;
-            // tournament over the rows j0..n-1
-            int nsets = (m + CH - 1) / CH;
-            int level = 0;
-            int* ci = cid;
-            T *vr = cvr, *vi = cvi;
-            int* co = cid + (size_t)n / PB * PB;  // second half of the candidate buffers
-            T *wr = cvr + (size_t)n * PB, *wi = cvi + (size_t)n * PB;
+            const int nsets = (m + CH - 1) / CH;
             int kt = kt_mark(c, KS_TSLU);
-            hipLaunchKernelGGL(k_tslu<T>, dim3(nsets), dim3(256), 0, s, 0, Ar, Ai, ld, j0, n, 0, nullptr, nullptr,
-                               nullptr, ci, vr, vi, nsets == 1 ? 1 : 0, c->ipiv);
-            while (nsets > 1) {
-                const int nout = (nsets + G - 1) / G;
-                level++;
-                hipLaunchKernelGGL(k_tslu<T>, dim3(nout), dim3(256), 0, s, level, Ar, Ai, ld, j0, n, nsets, ci, vr,
-                                   vi, co, wr, wi, nout == 1 ? 1 : 0, c->ipiv);
-                std::swap(ci, co);
-                std::swap(vr, wr);
-                std::swap(vi, wi);
-                nsets = nout;
-            }
+            hipLaunchKernelGGL(k_tslu_tree<T>, dim3(nsets), dim3(256), 0, s, Ar, Ai, ld, j0, n, cid, cvr, cvi, cnt,
+                               c->ipiv);
             kt_close(c, kt, 0.0, 2.0 * sizeof(T) * (double)m * PB);
             kt = kt_mark(c, KS_LASWP);
             hipLaunchKernelGGL(k_laswp<T>, dim3((n + 255) / 256), dim3(256), 0, s, Ar, Ai, ld, n, j0, PB, c->ipiv);
@@ -320,26 +362,37 @@ int factor(bem_ctx* c, T* Ar, T* Ai, int* dinfo, int* cid, T* cvr, T* cvi) {
             const int nblk = 1 + (m - PB + 255) / 256;
             kt = kt_mark(c, KS_PANEL);
             hipLaunchKernelGGL(k_panel<T>, dim3(nblk), dim3(256), 0, s, Ar, Ai, ld, j0, m, nright, dinfo);
-            kt_close(c, kt, 8.0 * (double)m * PB * PB / 2, 4.0 * sizeof(T) * (double)m * PB);
+            kt_close(c, kt, 4.0 * (double)m * PB * PB, 4.0 * sizeof(T) * (double)m * PB);
             if (nright > 0) {
                 const long o21 = (long)(j0 + PB) * ld + j0, o12 = (long)j0 * ld + j0 + PB;
                 const long o22 = (long)(j0 + PB) * ld + j0 + PB;
-                cgemm_ctx<T>(c, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22,
-                         ld, 0);
+                cgemm_ctx<T>(c, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22,
+                             Ai + o22, ld, 0);
             }
         }
+        // L11^{-1} of this diagonal block (kept for the solves)
+        const long o11 = (long)k0 * ld + k0;
+        const long io = (long)(k0 / NB) * NB * NB;
+        int kt = kt_mark(c, KS_TRSM);
+        invert_diag_blocks<T>(s, NB, 1, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, true);
+        kt_close(c, kt, 8.0 * NB * NB * NB / 6.0, 4.0 * sizeof(T) * (double)NB * NB);
         const int nr = n - (k0 + NB);
         if (nr > 0) {
-            const long o11 = (long)k0 * ld + k0, o12 = (long)k0 * ld + k0 + NB;
+            const long o12 = (long)k0 * ld + k0 + NB;
             const long o21 = (long)(k0 + NB) * ld + k0, o22 = (long)(k0 + NB) * ld + k0 + NB;
-            const int kt = kt_mark(c, KS_TRSM);
-            hipLaunchKernelGGL((k_trsm_lower_unit<T, 16>), dim3((nr + 15) / 16), dim3(256), 0, s, NB, Ar + o11,
-                               Ai + o11, ld, Ar + o12, Ai + o12, ld, nr);
-            kt_close(c, kt, 4.0 * (double)NB * NB * nr, 4.0 * sizeof(T) * (double)NB * nr);
-            cgemm_ctx<T>(c, nr, nr, NB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22, ld, 0);
+            // U12 = L11^{-1} A12 into the work buffer, trailing update, U12 back in place
+            cgemm_ctx<T>(c, NB, nr, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1);
+            cgemm_ctx<T>(c, nr, nr, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0);
+            kt = kt_mark(c, KS_OTHER);
+            hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)(((long)NB * nr + 255) / 256)), dim3(256), 0, s, NB,
+                               nr, ubr, ubi, ld, Ar + o12, Ai + o12, ld);
+            kt_close(c, kt, 0.0, 4.0 * sizeof(T) * (double)NB * nr);
         }
     }
-    return 0;
+    // U^{-1} of all diagonal blocks in one launch
+    const int kt = kt_mark(c, KS_TRSM);
+    invert_diag_blocks<T>(s, NB, n / NB, Ar, Ai, ld, Uir, Uii, false);
+    kt_close(c, kt, 8.0 * (double)n * NB * NB / 6.0, 4.0 * sizeof(T) * (double)n * NB);
 }
 
 template <typename T>
@@ -348,41 +401,89 @@ __global__ void k_convert(const double* __restrict__ a, T* __restrict__ b, size_
     if (i < n) b[i] = (T)a[i];
 }
 
+template <typename T>
+int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
+    const int n = c->npad;
+    // diagonal-block inverses: L and U, npad x NB per plane each
+    const size_t ib = 4 * (size_t)n * NB * sizeof(T);
+    if (inv_bytes < ib) {
+        if (inv) (void)hipFree(inv);
+        inv = nullptr;
+        inv_bytes = 0;
+        if (hipMalloc(&inv, ib) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "LU: diagonal-block inverses");
+        inv_bytes = ib;
+    }
+    T* Lir = static_cast<T*>(inv);
+    T* Lii = Lir + (size_t)n * NB;
+    T* Uir = Lii + (size_t)n * NB;
+    T* Uii = Uir + (size_t)n * NB;
+    // work: info + counters, candidate ids, candidate values, U12 buffer
+    const int nsets0 = (n + CH - 1) / CH;
+    const size_t ncnt = 2 * (size_t)nsets0 + 64;
+    const size_t bytes = 256 + ncnt * sizeof(int) + 2 * (size_t)nsets0 * PB * sizeof(int) +
+                         4 * (size_t)nsets0 * PB * PB * sizeof(T) + 2 * (size_t)NB * n * sizeof(T) + 256;
+    void* scr = nullptr;
+    if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
+    char* p = static_cast<char*>(scr);
+    int* dinfo = reinterpret_cast<int*>(p);
+    int* cnt = dinfo + 64;
+    int* cid = cnt + ncnt;
+    T* cvr = reinterpret_cast<T*>(cid + 2 * (size_t)nsets0 * PB);
+    T* cvi = cvr + 2 * (size_t)nsets0 * PB * PB;
+    T* ubr = cvi + 2 * (size_t)nsets0 * PB * PB;
+    T* ubi = ubr + (size_t)NB * n;
+    BEM_HIP(c, hipMemsetAsync(dinfo, 0, (64 + ncnt) * sizeof(int), c->stream));
+    factor<T>(c, Ar, Ai, Lir, Lii, Uir, Uii, dinfo, cid, cvr, cvi, cnt, ubr, ubi);
+    return 0;
+}
+
 }  // namespace
+
+// Diagonal-block inverses for FP64 factors loaded from the host (bem_set_lu).
+int lu_inverses_launch(bem_ctx* c) {
+    const int n = c->npad;
+    const size_t ib = 4 * (size_t)n * NB * sizeof(double);
+    if (c->inv64_bytes < ib) {
+        if (c->inv64) (void)hipFree(c->inv64);
+        c->inv64 = nullptr;
+        c->inv64_bytes = 0;
+        if (hipMalloc(&c->inv64, ib) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "LU: diagonal-block inverses");
+        c->inv64_bytes = ib;
+    }
+    double* Lir = static_cast<double*>(c->inv64);
+    double* Lii = Lir + (size_t)n * NB;
+    double* Uir = Lii + (size_t)n * NB;
+    double* Uii = Uir + (size_t)n * NB;
+    invert_diag_blocks<double>(c->stream, NB, n / NB, c->Vr, c->Vi, n, Lir, Lii, true);
+    invert_diag_blocks<double>(c->stream, NB, n / NB, c->Vr, c->Vi, n, Uir, Uii, false);
+    BEM_HIP(c, hipGetLastError());
+    BEM_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
 
 int lu_factor_launch(bem_ctx* c, int precision) {
     const int n = c->npad;
     const size_t nn = (size_t)n * n;
     hipStream_t s = c->stream;
-    // candidate buffers: 2 x (n/PB sets x PB ids) ints, 2 x (n x PB) values per plane, + info
-    const size_t esz = precision == BEM_F64 ? sizeof(double) : sizeof(float);
-    const size_t bytes = 2 * (size_t)n * sizeof(int) + 4 * (size_t)n * PB * esz + 2 * 256 + 64;
-    void* scr = nullptr;
-    if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
-    char* p = static_cast<char*>(scr);
-    int* dinfo = reinterpret_cast<int*>(p);
-    int* cid = dinfo + 64;
-    char* vals = reinterpret_cast<char*>(cid + 2 * (size_t)n);
-    BEM_HIP(c, hipMemsetAsync(dinfo, 0, sizeof(int), s));
+    int rc;
     if (precision == BEM_F64) {
         BEM_HIP(c, hipMemcpyAsync(c->Vr, c->Zr, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
         BEM_HIP(c, hipMemcpyAsync(c->Vi, c->Zi, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-        double* cvr = reinterpret_cast<double*>(vals);
-        double* cvi = cvr + 2 * (size_t)n * PB;
-        factor<double>(c, c->Vr, c->Vi, dinfo, cid, cvr, cvi);
+        rc = factor_prec<double>(c, c->Vr, c->Vi, c->inv64, c->inv64_bytes);
     } else {
         if (!c->has_Z32) {
             const size_t blocks = (nn + 255) / 256;
             hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, s, c->Zr, c->Wr, nn);
             hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, s, c->Zi, c->Wi, nn);
         }
-        float* cvr = reinterpret_cast<float*>(vals);
-        float* cvi = cvr + 2 * (size_t)n * PB;
-        factor<float>(c, c->Wr, c->Wi, dinfo, cid, cvr, cvi);
+        rc = factor_prec<float>(c, c->Wr, c->Wi, c->inv32, c->inv32_bytes);
     }
+    if (rc) return rc;
     BEM_HIP(c, hipGetLastError());
+    void* scr = nullptr;
+    if (int e = ctx_scratch(c, 256, &scr)) return e;
     int info = 0;
-    BEM_HIP(c, hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    BEM_HIP(c, hipMemcpyAsync(&info, scr, sizeof(int), hipMemcpyDeviceToHost, s));
     // swap sequence -> row permutation (row i of P Z is row perm[i] of Z)
     std::vector<int> ip(n), perm(n);
     BEM_HIP(c, hipMemcpyAsync(ip.data(), c->ipiv, n * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -391,5 +492,5 @@ int lu_factor_launch(bem_ctx* c, int precision) {
     for (int i = 0; i < n; i++)
         if (ip[i] != i) std::swap(perm[i], perm[ip[i]]);
     BEM_HIP(c, hipMemcpy(c->perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    return info > c->n ? 0 : info;  // zero pivots only in the identity padding cannot occur
+    return info > c->n ? 0 : info;  // a zero pivot inside the identity padding cannot occur
 }

@@ -64,34 +64,35 @@ __global__ __launch_bounds__(256) void k_colnorm2(int n, const double* __restric
     if (threadIdx.x == 0) out[j] = red[0];
 }
 
-// Y <- U^{-1} L^{-1} Y for the factors at (Fr, Fi), Y planar n x ncol (ld ldy).
+// Y <- U^{-1} L^{-1} Y for the factors at (Fr, Fi), Y planar n x ncol (ld ldy),
+// with the inverses of the 256 x 256 diagonal blocks at inv (L blocks, then U
+// blocks; each n x 256 per plane).  All products are matrix-core GEMMs:
+//   forward   W_I = Linv_I Y_I;   Y_{>I} -= L_{>I,I} W_I
+//   backward  Y_I = Uinv_I W_I;   W_{<I} -= U_{<I,I} Y_I
+// W is a work buffer of Y's shape; the result is left in Y.
 template <typename T>
-void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, T* Yr, T* Yi, int ncol, long ldy) {
+void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, const void* inv, T* Yr, T* Yi, T* Wr, T* Wi, int ncol, long ldy) {
     const int n = c->npad, nb = n / NB;
     const long ld = n;
-    hipStream_t s = c->stream;
-    const dim3 tg((ncol + 15) / 16);
+    const T* Lir = static_cast<const T*>(inv);
+    const T* Lii = Lir + (size_t)n * NB;
+    const T* Uir = Lii + (size_t)n * NB;
+    const T* Uii = Uir + (size_t)n * NB;
     for (int I = 0; I < nb; I++) {
-        const long d = (long)I * NB * ld + (long)I * NB, y = (long)I * NB * ldy;
-        int kt = kt_mark(c, KS_TRSM);
-        hipLaunchKernelGGL((k_trsm_lower_unit<T, 16>), tg, dim3(256), 0, s, NB, Fr + d, Fi + d, ld, Yr + y, Yi + y,
-                           ldy, ncol);
-        kt_close(c, kt, 4.0 * (double)NB * NB * ncol, 4.0 * sizeof(T) * (double)NB * ncol);
+        const long y = (long)I * NB * ldy, io = (long)I * NB * NB;
+        cgemm_ctx<T>(c, NB, ncol, NB, Lir + io, Lii + io, NB, Yr + y, Yi + y, ldy, Wr + y, Wi + y, ldy, 1);
         const int rest = n - (I + 1) * NB;
         if (rest > 0) {
             const long a = (long)(I + 1) * NB * ld + (long)I * NB, yc = (long)(I + 1) * NB * ldy;
-            cgemm_ctx<T>(c, rest, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr + yc, Yi + yc, ldy, 0);
+            cgemm_ctx<T>(c, rest, ncol, NB, Fr + a, Fi + a, ld, Wr + y, Wi + y, ldy, Yr + yc, Yi + yc, ldy, 0);
         }
     }
     for (int I = nb - 1; I >= 0; I--) {
-        const long d = (long)I * NB * ld + (long)I * NB, y = (long)I * NB * ldy;
-        int kt = kt_mark(c, KS_TRSM);
-        hipLaunchKernelGGL((k_trsm_upper<T, 16>), tg, dim3(256), 0, s, NB, Fr + d, Fi + d, ld, Yr + y, Yi + y, ldy,
-                           ncol);
-        kt_close(c, kt, 4.0 * (double)NB * NB * ncol, 4.0 * sizeof(T) * (double)NB * ncol);
+        const long y = (long)I * NB * ldy, io = (long)I * NB * NB;
+        cgemm_ctx<T>(c, NB, ncol, NB, Uir + io, Uii + io, NB, Wr + y, Wi + y, ldy, Yr + y, Yi + y, ldy, 1);
         if (I > 0) {
             const long a = (long)I * NB;  // rows 0 .. I*NB-1, columns of block I
-            cgemm_ctx<T>(c, I * NB, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Yr, Yi, ldy, 0);
+            cgemm_ctx<T>(c, I * NB, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Wr, Wi, ldy, 0);
         }
     }
 }
@@ -114,15 +115,16 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
     hipStream_t s = c->stream;
     const bool f32 = c->factored == BEM_F32_3M;
     const bool can_refine = c->has_Z64;
-    // scratch: R (FP64 planar), Y (working precision planar), norms
+    // scratch: R (FP64 planar), Y and W (working precision planar), norms
     const size_t ysz = f32 ? sizeof(float) : sizeof(double);
-    const size_t bytes = 2 * cnt * sizeof(double) + 2 * cnt * ysz + 2 * (size_t)ldr * sizeof(double) + 256;
+    const size_t bytes = 2 * cnt * sizeof(double) + 4 * cnt * ysz + 2 * (size_t)ldr * sizeof(double) + 256;
     void* scr = nullptr;
     if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
     double* Rr = static_cast<double*>(scr);
     double* Ri = Rr + cnt;
     char* yp = reinterpret_cast<char*>(Ri + cnt);
-    double* nb2 = reinterpret_cast<double*>(yp + 2 * cnt * ysz);
+    char* wp = yp + 2 * cnt * ysz;
+    double* nb2 = reinterpret_cast<double*>(wp + 2 * cnt * ysz);
     double* nr2 = nb2 + ldr;
     hipEvent_t e[4];
     for (auto& x : e) BEM_HIP(c, hipEventCreate(&x));
@@ -135,16 +137,20 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
         if (f32) {
             float* Yr = reinterpret_cast<float*>(yp);
             float* Yi = Yr + cnt;
+            float* Wr = reinterpret_cast<float*>(wp);
+            float* Wi = Wr + cnt;
             hipLaunchKernelGGL(k_gather_rows<float>, gr, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ldr, Yr, Yi,
                                (long)ldr);
-            lu_apply<float>(c, c->Wr, c->Wi, Yr, Yi, ncol, ldr);
+            lu_apply<float>(c, c->Wr, c->Wi, c->inv32, Yr, Yi, Wr, Wi, ncol, ldr);
             hipLaunchKernelGGL(k_accumulate<float>, g1, dim3(256), 0, s, cnt, Yr, Yi, c->Xr, c->Xi, assign);
         } else {
             double* Yr = reinterpret_cast<double*>(yp);
             double* Yi = Yr + cnt;
+            double* Wr = reinterpret_cast<double*>(wp);
+            double* Wi = Wr + cnt;
             hipLaunchKernelGGL(k_gather_rows<double>, gr, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ldr, Yr,
                                Yi, (long)ldr);
-            lu_apply<double>(c, c->Vr, c->Vi, Yr, Yi, ncol, ldr);
+            lu_apply<double>(c, c->Vr, c->Vi, c->inv64, Yr, Yi, Wr, Wi, ncol, ldr);
             hipLaunchKernelGGL(k_accumulate<double>, g1, dim3(256), 0, s, cnt, Yr, Yi, c->Xr, c->Xi, assign);
         }
     };
