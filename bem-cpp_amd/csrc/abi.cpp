@@ -33,6 +33,7 @@ int ctx_scratch(bem_ctx* c, size_t bytes, void** out) {
 void kt_resolve(bem_ctx* c) {
     KernelTimer& k = c->kt;
     if (k.pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream);
     for (KernelTimer::Rec& r : k.pending) {
         float ms = 0.0f;
@@ -186,7 +187,15 @@ bem_ctx* bem_create(int device) {
     bem_ctx* c = new bem_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess) {
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
+        hipEventCreateWithFlags(&c->lu_ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->lu_ev[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->lu_ev[2], hipEventDisableTiming) != hipSuccess) {
+        for (hipEvent_t e : {c->ev[0], c->ev[1], c->lu_ev[0], c->lu_ev[1], c->lu_ev[2]})
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t s : {c->stream, c->stream2})
+            if (s) (void)hipStreamDestroy(s);
         delete c;
         return nullptr;
     }
@@ -196,6 +205,7 @@ bem_ctx* bem_create(int device) {
 void bem_destroy(bem_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream);
     free_mesh(c);
     free_matrices(c);
@@ -204,6 +214,8 @@ void bem_destroy(bem_ctx* c) {
     if (c->scratch) (void)hipFree(c->scratch);
     (void)hipEventDestroy(c->ev[0]);
     (void)hipEventDestroy(c->ev[1]);
+    for (hipEvent_t e : c->lu_ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -54,6 +54,7 @@ struct KernelTimer {
     bool on = false;
     struct Rec {
         hipEvent_t a, b;
+        hipStream_t s;
         int cls;
         double flops, bytes;
     };
@@ -67,7 +68,9 @@ struct bem_ctx {
     int device = 0;
     KernelTimer kt;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // panel stream of the look-ahead factorisation
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t lu_ev[3] = {nullptr, nullptr, nullptr};  // cross-stream ordering inside the factorisation
     std::string err;
 
     // mesh and basis (host copies)
@@ -150,8 +153,9 @@ int ctx_ensure_rhs(bem_ctx* c, int nrhs);
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // Kernel timing helpers (no-ops unless enabled).  kt_mark returns a record
-// index to close with kt_close after the launch.
-inline int kt_mark(bem_ctx* c, int cls) {
+// index to close with kt_close after the launch; both events go on stream s
+// (default: the context stream).
+inline int kt_mark(bem_ctx* c, int cls, hipStream_t s = nullptr) {
     KernelTimer& k = c->kt;
     if (!k.on) return -1;
     KernelTimer::Rec r{};
@@ -164,7 +168,8 @@ inline int kt_mark(bem_ctx* c, int cls) {
         }
     }
     r.cls = cls;
-    (void)hipEventRecord(r.a, c->stream);
+    r.s = s ? s : c->stream;
+    (void)hipEventRecord(r.a, r.s);
     k.pending.push_back(r);
     return (int)k.pending.size() - 1;
 }
@@ -173,7 +178,7 @@ inline void kt_close(bem_ctx* c, int idx, double flops, double bytes) {
     KernelTimer::Rec& r = c->kt.pending[idx];
     r.flops = flops;
     r.bytes = bytes;
-    (void)hipEventRecord(r.b, c->stream);
+    (void)hipEventRecord(r.b, r.s);
 }
 // Waits for the recorded launches and folds them into the totals.
 void kt_resolve(bem_ctx* c);

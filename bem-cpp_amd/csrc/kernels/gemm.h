@@ -226,15 +226,16 @@ inline void cgemm(hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, 
     hipLaunchKernelGGL(k_cgemm<T>, grid, dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
 }
 
-// GEMM with optional per-launch timing on the context stream.  Counted work:
+// GEMM with optional per-launch timing, on stream s (default: the context stream).  Counted work:
 // 3M form 6 M N K real flops (FP32), 4M form 8 M N K (FP64); bytes = operands
 // read once plus C read and written.
 template <typename T>
 inline void cgemm_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br, const T* Bi,
-                      long ldb, T* Cr, T* Ci, long ldc, int mode) {
+                      long ldb, T* Cr, T* Ci, long ldc, int mode, hipStream_t s = nullptr) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64);
-    cgemm<T>(c->stream, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+    if (!s) s = c->stream;
+    const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, s);
+    cgemm<T>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
     const double mnk = (double)M * N * K;
     const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
     kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);

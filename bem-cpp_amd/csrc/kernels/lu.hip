@@ -39,121 +39,216 @@ __device__ __forceinline__ T mag2(T r, T i) {
     return r * r + i * i;
 }
 
-// Block-wide argmax of (val, idx) over 256 threads; ties -> smaller idx.
+// 16-byte vector type of T (4 floats or 2 doubles).
 template <typename T>
-__device__ __forceinline__ void argmax256(T val, int idx, T* sv, int* si, T& best, int& bidx) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const T ov = __shfl_xor(val, off);
-        const int oi = __shfl_xor(idx, off);
-        if (ov > val || (ov == val && oi < idx)) {
-            val = ov;
-            idx = oi;
-        }
-    }
-    if (lane == 0) {
-        sv[wave] = val;
-        si[wave] = idx;
-    }
-    __syncthreads();
-    best = sv[0];
-    bidx = si[0];
-    for (int w = 1; w < 4; w++)
-        if (sv[w] > best || (sv[w] == best && si[w] < bidx)) {
-            best = sv[w];
-            bidx = si[w];
-        }
-    __syncthreads();
-}
-
-template <typename T>
-struct TsluShared {
-    T vr[CH][PB + 1], vi[CH][PB + 1];
-    int id[CH], src[CH];
-    T sv[4];
-    int si[4];
-    int last;
+struct Vec16;
+template <>
+struct Vec16<float> {
+    using type = float4;
+};
+template <>
+struct Vec16<double> {
+    using type = double2;
 };
 
-// Partial pivoting on the nrows rows in LDS: rows 0..31 end up as the chosen
-// rows in pivot order (id/src permuted along).
+// PB consecutive values at p (16-byte aligned) <-> registers.
 template <typename T>
-__device__ void gepp_select(TsluShared<T>& S, int nrows) {
-    const int t = threadIdx.x;
+__device__ __forceinline__ void load_row(const T* __restrict__ p, T (&x)[PB]) {
+    using V = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < PB; q += E) {
+        const V a = *reinterpret_cast<const V*>(p + q);
+        __builtin_memcpy(&x[q], &a, 16);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void store_row(T* __restrict__ p, const T (&x)[PB]) {
+    using V = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < PB; q += E) {
+        V a;
+        __builtin_memcpy(&a, &x[q], 16);
+        *reinterpret_cast<V*>(p + q) = a;
+    }
+}
+
+// Exchange area of the register-resident elimination (double-buffered by
+// step parity so each step needs one barrier).
+template <typename T>
+struct ElimLds {
+    T pr[2][4][PB], pi[2][4][PB];  // best row of each wave
+    T bv[2][4];
+    int bp[2][4];
+    int oid[PB], osrc[PB];  // chosen rows in pivot order
+    int last;
+    int zmin;  // first zero pivot of the root (1-based)
+};
+
+// Partial pivoting over the block's rows, one row per thread in registers
+// (rows t >= nrows take no part), with implicit row interchanges: rows stay
+// with their thread and a chosen row is only marked.  Step jc: the largest
+// |x_jc|^2 among the rows not chosen yet (ties: lower thread) becomes pivot
+// jc, and the remaining rows are eliminated (multiplier stored in x_jc).
+// Afterwards the thread of pivot r holds row r of L11 \ U11 and returns r
+// (-1 for rows not chosen); oid/osrc list the chosen rows in pivot order.
+template <typename T>
+__device__ __forceinline__ int elim_rows(ElimLds<T>& L, T (&xr)[PB], T (&xi)[PB], int id, int src, int nrows) {
+    const int t = threadIdx.x, wave = t >> 6;
+    const bool valid = t < nrows;
+    int mine = -1;
+#pragma unroll
     for (int jc = 0; jc < PB; jc++) {
-        const T v = (t >= jc && t < nrows) ? mag2(S.vr[t][jc], S.vi[t][jc]) : T(-1);
-        T best;
-        int p;
-        argmax256(v, t, S.sv, S.si, best, p);
-        if (p != jc && t < PB) {
-            T a = S.vr[jc][t];
-            S.vr[jc][t] = S.vr[p][t];
-            S.vr[p][t] = a;
-            a = S.vi[jc][t];
-            S.vi[jc][t] = S.vi[p][t];
-            S.vi[p][t] = a;
-        }
-        if (p != jc && t == 0) {
-            int x = S.id[jc];
-            S.id[jc] = S.id[p];
-            S.id[p] = x;
-            x = S.src[jc];
-            S.src[jc] = S.src[p];
-            S.src[p] = x;
-        }
-        __syncthreads();
-        if (best > T(0) && t > jc && t < nrows) {
-            T lr, li;
-            cdiv(S.vr[t][jc], S.vi[t][jc], S.vr[jc][jc], S.vi[jc][jc], lr, li);
-            for (int c = jc + 1; c < PB; c++) {
-                const T ur = S.vr[jc][c], ui = S.vi[jc][c];
-                S.vr[t][c] -= lr * ur - li * ui;
-                S.vi[t][c] -= lr * ui + li * ur;
+        const int b = jc & 1;
+        const bool open = valid && mine < 0;
+        T v = open ? mag2(xr[jc], xi[jc]) : T(-1);
+        int p = t;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(v, off);
+            const int op = __shfl_xor(p, off);
+            if (ov > v || (ov == v && op < p)) {
+                v = ov;
+                p = op;
             }
         }
+        if (t == p) {
+#pragma unroll
+            for (int c = 0; c < PB; c++)
+                if (c >= jc) {
+                    L.pr[b][wave][c] = xr[c];
+                    L.pi[b][wave][c] = xi[c];
+                }
+            L.bv[b][wave] = v;
+            L.bp[b][wave] = p;
+        }
         __syncthreads();
+        T best = L.bv[b][0];
+        int pp = L.bp[b][0], pw = 0;
+#pragma unroll
+        for (int w = 1; w < 4; w++) {
+            const T bw = L.bv[b][w];
+            const int qw = L.bp[b][w];
+            if (bw > best || (bw == best && qw < pp)) {
+                best = bw;
+                pp = qw;
+                pw = w;
+            }
+        }
+        if (t == pp) {
+            mine = jc;
+        } else if (open && best > T(0)) {
+            T lr, li;
+            cdiv(xr[jc], xi[jc], L.pr[b][pw][jc], L.pi[b][pw][jc], lr, li);
+            xr[jc] = lr;
+            xi[jc] = li;
+#pragma unroll
+            for (int c = 0; c < PB; c++)
+                if (c > jc) {
+                    const T ur = L.pr[b][pw][c], ui = L.pi[b][pw][c];
+                    xr[c] -= lr * ur - li * ui;
+                    xi[c] -= lr * ui + li * ur;
+                }
+        }
     }
+    if (mine >= 0) {
+        L.oid[mine] = id;
+        L.osrc[mine] = src;
+    }
+    __syncthreads();
+    return mine;
+}
+
+// Root of the tournament, one wave: pivot order -> LAPACK-style swaps
+// ipiv[j0 .. j0+31], and their net effect on the touched rows as a list of
+// (destination, source) row pairs: pl[0] = count, pl[1 + 2k] = destination,
+// pl[2 + 2k] = source.  Lane l tracks one touched row (<= 64 rows).
+__device__ void root_swaps(const int* __restrict__ oid, int j0, int* __restrict__ ipiv, int* __restrict__ pl) {
+    const int lane = threadIdx.x;  // wave 0
+    // pos: lane c < PB holds the current location of the c-th chosen row
+    int pos = lane < PB ? oid[lane] : -1;
+    int rows = -1, cont = -1, nt = 0;
+    for (int c = 0; c < PB; c++) {
+        const int x = j0 + c;
+        const int y = __shfl(pos, c);
+        if (lane == 0) ipiv[x] = y;
+        if (lane > c && pos == x) pos = y;  // the row at x moves to y
+        if (y == x) continue;
+        // net permutation of the touched rows
+        unsigned long long m = __ballot(rows == x);
+        int ix;
+        if (m == 0ull) {
+            ix = nt++;
+            if (lane == ix) rows = cont = x;
+        } else {
+            ix = __ffsll((long long)m) - 1;
+        }
+        m = __ballot(rows == y);
+        int iy;
+        if (m == 0ull) {
+            iy = nt++;
+            if (lane == iy) rows = cont = y;
+        } else {
+            iy = __ffsll((long long)m) - 1;
+        }
+        const int cx = __shfl(cont, ix), cy = __shfl(cont, iy);
+        if (lane == ix) cont = cy;
+        if (lane == iy) cont = cx;
+    }
+    const bool moved = lane < nt && cont != rows;
+    const unsigned long long mm = __ballot(moved);
+    const int k = __popcll(mm & ((1ull << lane) - 1ull));
+    if (moved) {
+        pl[1 + 2 * k] = rows;
+        pl[2 + 2 * k] = cont;
+    }
+    if (lane == 0) pl[0] = __popcll(mm);
 }
 
 // ---------------------------------------------------------------------------
 // Pivot tournament for columns j0..j0+31 over rows j0..n-1, one launch.
-// Candidate sets of level L live at set index off[L] + i; set = 32 ids +
-// 32 x 32 original values per plane.  cnt: one arrival counter per group per
-// level (zero on entry; reset by the merging block).
+// Candidate sets of level L live at set index off[L] + i; a set is 32 ids and
+// the 32 x 32 original values per plane.  cnt: one arrival counter per group
+// per level (zero on entry; reset by the merging block).  The root writes
+// ipiv, the row-pair list pl, the factored diagonal block d11 (32 x 32 per
+// plane) and the first zero pivot into *info.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_tslu_tree(const T* __restrict__ Ar, const T* __restrict__ Ai, long ld,
-                                                   int j0, int n, int* __restrict__ cid, T* __restrict__ cvr,
-                                                   T* __restrict__ cvi, int* __restrict__ cnt, int* __restrict__ ipiv) {
-    __shared__ TsluShared<T> S;
+__global__ __launch_bounds__(256) void k_tslu(const T* __restrict__ Ar, const T* __restrict__ Ai, long ld, int j0,
+                                              int n, int* __restrict__ cid, T* __restrict__ cvr, T* __restrict__ cvi,
+                                              int* __restrict__ cnt, int* __restrict__ ipiv, int* __restrict__ pl,
+                                              T* __restrict__ d11r, T* __restrict__ d11i, int* __restrict__ info) {
+    __shared__ ElimLds<T> L;
     const int t = threadIdx.x;
-    // level 0: rows j0 + b*CH .. (coalesced: 32 consecutive columns of a row)
+    // level 0: rows j0 + b*CH .. (n - j0 is a multiple of PB, so every set is full)
     const int r0 = j0 + blockIdx.x * CH;
     int nrows = min(CH, n - r0);
-    for (int e = t; e < nrows * PB; e += 256) {
-        const int r = e / PB, c = e % PB;
-        const long o = (long)(r0 + r) * ld + j0 + c;
-        S.vr[r][c] = Ar[o];
-        S.vi[r][c] = Ai[o];
-    }
+    T xr[PB], xi[PB];
     if (t < nrows) {
-        S.id[t] = r0 + t;
-        S.src[t] = r0 + t;
+        load_row(Ar + (long)(r0 + t) * ld + j0, xr);
+        load_row(Ai + (long)(r0 + t) * ld + j0, xi);
     }
-    __syncthreads();
-    gepp_select(S, nrows);
-    // write candidate set = original rows of the chosen ids
+    int mine = elim_rows(L, xr, xi, r0 + t, r0 + t, nrows);
     int level_sets = (n - j0 + CH - 1) / CH;  // sets at this level
     int set = blockIdx.x, off = 0, cnt_off = 0;
-    for (int e = t; e < PB * PB; e += 256) {
-        const int r = e / PB, c = e % PB;
-        const long g = (long)S.src[r] * ld + j0 + c;
-        cvr[((long)(off + set) * PB + r) * PB + c] = Ar[g];
-        cvi[((long)(off + set) * PB + r) * PB + c] = Ai[g];
-    }
-    if (t < PB) cid[(off + set) * PB + t] = S.id[t];
+    bool from_matrix = true;
     while (level_sets > 1) {
+        // candidate set = original values of the chosen rows
+        for (int e = t; e < PB * PB; e += 256) {
+            const int r = e / PB, c = e % PB;
+            const long o = ((long)(off + set) * PB + r) * PB + c;
+            if (from_matrix) {
+                const long g = (long)L.osrc[r] * ld + j0 + c;
+                cvr[o] = Ar[g];
+                cvi[o] = Ai[g];
+            } else {
+                const long g = (long)L.osrc[r] * PB + c;
+                cvr[o] = cvr[g];
+                cvi[o] = cvi[g];
+            }
+        }
+        if (t < PB) cid[(off + set) * PB + t] = L.oid[t];
         // arrive at the group counter; the last arrival merges the group
         const int grp = set / G;
         const int gsize = min(G, level_sets - grp * G);
@@ -161,171 +256,165 @@ __global__ __launch_bounds__(256) void k_tslu_tree(const T* __restrict__ Ar, con
         __syncthreads();
         if (t == 0) {
             const int prev = atomicAdd(&cnt[cnt_off + grp], 1);
-            S.last = prev == gsize - 1;
-            if (S.last) cnt[cnt_off + grp] = 0;  // ready for the next launch
+            L.last = prev == gsize - 1;
+            if (L.last) cnt[cnt_off + grp] = 0;  // ready for the next launch
         }
         __syncthreads();
-        if (!S.last) return;
+        if (!L.last) return;
         __threadfence();
-        // load the group's candidate sets (original values)
         const int s0 = off + grp * G;
         nrows = gsize * PB;
-        for (int e = t; e < nrows * PB; e += 256) {
-            const int r = e / PB, c = e % PB;
-            S.vr[r][c] = cvr[((long)s0 * PB + r) * PB + c];
-            S.vi[r][c] = cvi[((long)s0 * PB + r) * PB + c];
-        }
+        int id = -1;
         if (t < nrows) {
-            S.id[t] = cid[s0 * PB + t];
-            S.src[t] = s0 * PB + t;
+            load_row(cvr + ((long)s0 * PB + t) * PB, xr);
+            load_row(cvi + ((long)s0 * PB + t) * PB, xi);
+            id = cid[s0 * PB + t];
         }
-        __syncthreads();
-        gepp_select(S, nrows);
-        const int noff = off + level_sets;
+        mine = elim_rows(L, xr, xi, id, s0 * PB + t, nrows);
+        from_matrix = false;
         cnt_off += (level_sets + G - 1) / G;
+        off += level_sets;
         set = grp;
-        off = noff;
         level_sets = (level_sets + G - 1) / G;
-        for (int e = t; e < PB * PB; e += 256) {
-            const int r = e / PB, c = e % PB;
-            const long g = (long)S.src[r] * PB + c;
-            cvr[((long)(off + set) * PB + r) * PB + c] = cvr[g];
-            cvi[((long)(off + set) * PB + r) * PB + c] = cvi[g];
-        }
-        if (t < PB) cid[(off + set) * PB + t] = S.id[t];
     }
-    // root: pivot order -> sequential swaps; a chosen row sitting at the
-    // position being vacated moves to the other side of the swap
-    if (t == 0) {
-        int pos[PB];
-        for (int c = 0; c < PB; c++) pos[c] = S.id[c];
-        for (int c = 0; c < PB; c++) {
-            const int x = j0 + c, y = pos[c];
-            ipiv[x] = y;
-            if (y != x)
-                for (int c2 = c + 1; c2 < PB; c2++)
-                    if (pos[c2] == x) pos[c2] = y;
-            pos[c] = x;
-        }
+    // root: the factored diagonal block, swaps, row-pair list, zero pivot
+    if (t == 0) L.zmin = 0x7fffffff;
+    __syncthreads();
+    if (mine >= 0) {
+        store_row(d11r + mine * PB, xr);
+        store_row(d11i + mine * PB, xi);
+        T dr = T(0), di = T(0);
+#pragma unroll
+        for (int c = 0; c < PB; c++)
+            if (c == mine) {
+                dr = xr[c];
+                di = xi[c];
+            }
+        if (dr == T(0) && di == T(0)) atomicMin(&L.zmin, j0 + mine + 1);
     }
+    __syncthreads();
+    if (t == 0 && L.zmin != 0x7fffffff && *info == 0) *info = L.zmin;
+    if (t < 64) root_swaps(L.oid, j0, ipiv, pl);
 }
 
-// Applies the swaps ipiv[r0 .. r0+cnt) to every column (one thread per column).
+// Applies nl row-pair lists in order (list l at pl + l * PLS, each the net
+// effect of one panel step's swaps) to the columns [ca, cb) and [cc, cd): per
+// list, all sources are read into LDS before any destination is written.
+// 64 columns per block, 4 row groups.
+constexpr int PLS = 256;  // ints per row-pair list
 template <typename T>
-__global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict__ Ai, long ld, int ncols, int r0,
-                                               int cnt, const int* __restrict__ ipiv) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
-    for (int k = 0; k < cnt; k++) {
-        const int x = r0 + k, y = ipiv[x];
-        if (y == x) continue;
-        const long ox = (long)x * ld + c, oy = (long)y * ld + c;
-        const T a = Ar[ox], b = Ai[ox];
-        Ar[ox] = Ar[oy];
-        Ai[ox] = Ai[oy];
-        Ar[oy] = a;
-        Ai[oy] = b;
+__global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict__ Ai, long ld, int ca, int cb, int cc,
+                                               int cd, const int* __restrict__ pl, int nl) {
+    __shared__ T br[2 * PB][64], bi[2 * PB][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int v = blockIdx.x * 64 + tx;  // virtual column over both ranges
+    const int w1 = cb - ca;
+    const int c = v < w1 ? ca + v : cc + (v - w1);
+    const bool ok = v < w1 + (cd - cc);
+    for (int l = 0; l < nl; l++) {
+        const int* q = pl + l * PLS;
+        const int np = q[0];
+        if (np == 0) continue;
+        for (int k = ty; k < np; k += 4)
+            if (ok) {
+                const long o = (long)q[2 + 2 * k] * ld + c;
+                br[k][tx] = Ar[o];
+                bi[k][tx] = Ai[o];
+            }
+        __syncthreads();
+        for (int k = ty; k < np; k += 4)
+            if (ok) {
+                const long o = (long)q[1 + 2 * k] * ld + c;
+                Ar[o] = br[k][tx];
+                Ai[o] = bi[k][tx];
+            }
+        __syncthreads();
     }
 }
 
-// Panel step after the swaps.  Every block factors the 32 x 32 diagonal block
-// (rows/cols j0..j0+31) in LDS without pivoting.  Block 0 writes it back,
-// computes U12 = L11^{-1} A12 for columns j0+32 .. j0+32+nright and records a
-// zero pivot in *info.  Blocks >= 1 compute L21 = A21 U11^{-1} for 256 rows
-// each, staged through LDS with coalesced loads and stores.
+// Panel step after the swaps, with the factored 32 x 32 diagonal block d11
+// from the tournament.  Block 0 writes d11 in place and forms U12 = L11^{-1}
+// A12 for the nright columns right of the block (one column per thread, in
+// registers).  Blocks >= 1 form L21 = A21 U11^{-1} for 256 rows each (one row
+// per thread, in registers).  A zero pivot leaves its column unscaled.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict__ Ai, long ld, int j0, int nrows,
-                                               int nright, int* __restrict__ info) {
+                                               int nright, const T* __restrict__ d11r,
+                                               const T* __restrict__ d11i) {
     __shared__ T ur[PB][PB + 1], ui[PB][PB + 1];
-    __shared__ T lr_s[256][PB + 1], li_s[256][PB + 1];
+    __shared__ T rr[PB], ri[PB];
     const int t = threadIdx.x;
     for (int e = t; e < PB * PB; e += 256) {
         const int r = e / PB, c = e % PB;
-        ur[r][c] = Ar[(long)(j0 + r) * ld + j0 + c];
-        ui[r][c] = Ai[(long)(j0 + r) * ld + j0 + c];
+        ur[r][c] = d11r[e];
+        ui[r][c] = d11i[e];
     }
-    const int rb = j0 + PB + (blockIdx.x - 1) * 256;  // first L21 row of this block
-    const int nr = blockIdx.x == 0 ? 0 : min(256, j0 + nrows - rb);
-    for (int e = t; e < nr * PB; e += 256) {
-        const int r = e / PB, c = e % PB;
-        lr_s[r][c] = Ar[(long)(rb + r) * ld + j0 + c];
-        li_s[r][c] = Ai[(long)(rb + r) * ld + j0 + c];
+    if (t < PB) {
+        const T a = d11r[t * PB + t], b = d11i[t * PB + t];
+        if (a == T(0) && b == T(0)) {
+            rr[t] = T(1);
+            ri[t] = T(0);
+        } else {
+            cdiv(T(1), T(0), a, b, rr[t], ri[t]);
+        }
     }
     __syncthreads();
-    for (int k = 0; k < PB; k++) {
-        const T pr = ur[k][k], pi = ui[k][k];
-        const bool zero = pr == T(0) && pi == T(0);
-        if (!zero && t > k && t < PB) {
-            T a, b;
-            cdiv(ur[t][k], ui[t][k], pr, pi, a, b);
-            ur[t][k] = a;
-            ui[t][k] = b;
-            for (int c = k + 1; c < PB; c++) {
-                const T x = ur[k][c], y = ui[k][c];
-                ur[t][c] -= a * x - b * y;
-                ui[t][c] -= a * y + b * x;
-            }
-        }
-        __syncthreads();
-    }
     if (blockIdx.x == 0) {
         for (int e = t; e < PB * PB; e += 256) {
             const int r = e / PB, c = e % PB;
             Ar[(long)(j0 + r) * ld + j0 + c] = ur[r][c];
             Ai[(long)(j0 + r) * ld + j0 + c] = ui[r][c];
         }
-        if (t == 0) {
-            for (int k = 0; k < PB; k++)
-                if (ur[k][k] == T(0) && ui[k][k] == T(0)) {
-                    if (*info == 0) *info = j0 + k + 1;
-                    break;
-                }
-        }
-        // U12 of the rest of the outer panel: unit-lower forward substitution,
-        // one column per thread, the column kept in this thread's row of lr_s
-        // (block 0 stages no L21 rows)
         for (int c = t; c < nright; c += 256) {
             const long base = (long)j0 * ld + j0 + PB + c;
+            T xr[PB], xi[PB];
+#pragma unroll
             for (int r = 0; r < PB; r++) {
-                lr_s[t][r] = Ar[base + (long)r * ld];
-                li_s[t][r] = Ai[base + (long)r * ld];
+                xr[r] = Ar[base + (long)r * ld];
+                xi[r] = Ai[base + (long)r * ld];
             }
-            for (int l = 0; l < PB - 1; l++) {
-                const T xr = lr_s[t][l], xi = li_s[t][l];
-                for (int r = l + 1; r < PB; r++) {
-                    const T a = ur[r][l], b = ui[r][l];
-                    lr_s[t][r] -= a * xr - b * xi;
-                    li_s[t][r] -= a * xi + b * xr;
-                }
+#pragma unroll
+            for (int r = 1; r < PB; r++) {
+#pragma unroll
+                for (int l = 0; l < PB; l++)
+                    if (l < r) {
+                        const T a = ur[r][l], b = ui[r][l];
+                        xr[r] -= a * xr[l] - b * xi[l];
+                        xi[r] -= a * xi[l] + b * xr[l];
+                    }
+                __builtin_amdgcn_sched_barrier(0);  // keeps the LDS reads of L from being hoisted
             }
+#pragma unroll
             for (int r = 0; r < PB; r++) {
-                Ar[base + (long)r * ld] = lr_s[t][r];
-                Ai[base + (long)r * ld] = li_s[t][r];
+                Ar[base + (long)r * ld] = xr[r];
+                Ai[base + (long)r * ld] = xi[r];
             }
         }
         return;
     }
-    // L21 rows: x U11 = a  ->  x_c = (a_c - sum_{l<c} x_l U[l][c]) / U[c][c]
-    if (t < nr) {
-        for (int c = 0; c < PB; c++) {
-            T sr = lr_s[t][c], si = li_s[t][c];
-            for (int l = 0; l < c; l++) {
-                const T xr = lr_s[t][l], xi = li_s[t][l];
-                sr -= xr * ur[l][c] - xi * ui[l][c];
-                si -= xr * ui[l][c] + xi * ur[l][c];
+    const int row = j0 + PB + (blockIdx.x - 1) * 256 + t;
+    if (row >= j0 + nrows) return;
+    T xr[PB], xi[PB];
+    T* pr = Ar + (long)row * ld + j0;
+    T* pi = Ai + (long)row * ld + j0;
+    load_row(pr, xr);
+    load_row(pi, xi);
+#pragma unroll
+    for (int c = 0; c < PB; c++) {
+        T sr = xr[c], si = xi[c];
+#pragma unroll
+        for (int l = 0; l < PB; l++)
+            if (l < c) {
+                const T a = ur[l][c], b = ui[l][c];
+                sr -= xr[l] * a - xi[l] * b;
+                si -= xr[l] * b + xi[l] * a;
             }
-            T qr = sr, qi = si;
-            if (!(ur[c][c] == T(0) && ui[c][c] == T(0))) cdiv(sr, si, ur[c][c], ui[c][c], qr, qi);
-            lr_s[t][c] = qr;
-            li_s[t][c] = qi;
-        }
+        xr[c] = sr * rr[c] - si * ri[c];
+        xi[c] = sr * ri[c] + si * rr[c];
+        __builtin_amdgcn_sched_barrier(0);  // keeps the LDS reads of U from being hoisted
     }
-    __syncthreads();
-    for (int e = t; e < nr * PB; e += 256) {
-        const int r = e / PB, c = e % PB;
-        Ar[(long)(rb + r) * ld + j0 + c] = lr_s[r][c];
-        Ai[(long)(rb + r) * ld + j0 + c] = li_s[r][c];
-    }
+    store_row(pr, xr);
+    store_row(pi, xi);
 }
 
 template <typename T>
@@ -339,59 +428,116 @@ __global__ __launch_bounds__(256) void k_copy_rows(int rows, int cols, const T* 
     Di[(long)r * ldd + c] = Si[(long)r * lds + c];
 }
 
-// Blocked factorisation of the padded npad x npad matrix at (Ar, Ai).
-// work: candidate sets (ids, values), arrival counters, a 256 x n U12 buffer.
+// Work buffers of one factorisation (device).
 template <typename T>
-void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, int* dinfo, int* cid, T* cvr, T* cvi, int* cnt,
-            T* ubr, T* ubi) {
+struct LuWork {
+    int* info;       // first zero pivot (1-based), 0 if none
+    int* cnt;        // tournament arrival counters
+    int* cid;        // candidate row ids
+    T *cvr, *cvi;    // candidate values
+    int* pl;         // row-pair lists of the 8 steps of the current outer panel (PLS ints each)
+    T *d11r, *d11i;  // factored 32 x 32 diagonal block
+    T *ubr, *ubi;    // 256 x n U12 buffer
+};
+
+template <typename T>
+void launch_laswp(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, long ld, int ca, int cb, int cc, int cd, const int* pl,
+                  int nl) {
+    const int ncols = (cb - ca) + (cd - cc);
+    if (ncols <= 0) return;
+    const int kt = kt_mark(c, KS_LASWP, s);
+    hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 63) / 64), dim3(256), 0, s, Ar, Ai, ld, ca, cb, cc, cd, pl, nl);
+    kt_close(c, kt, 0.0, 2.0 * 2.0 * sizeof(T) * (double)ncols * 2 * PB * nl);
+}
+
+// Factors the outer panel of columns k0 .. k0+NB-1 (rows k0 .. n-1) on stream
+// s: 8 steps of tournament, swaps within the panel's columns, L21/U12 of the
+// step and the rank-32 update of the panel's remaining columns.  The swaps of
+// step j are kept as row-pair list j for the columns outside the panel.
+template <typename T>
+void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<T>& w) {
     const int n = c->npad;
     const long ld = n;
-    hipStream_t s = c->stream;
-    for (int k0 = 0; k0 < n; k0 += NB) {
-        for (int j0 = k0; j0 < k0 + NB; j0 += PB) {
-            const int m = n - j0;
-            const int nsets = (m + CH - 1) / CH;
-            int kt = kt_mark(c, KS_TSLU);
-            hipLaunchKernelGGL(k_tslu_tree<T>, dim3(nsets), dim3(256), 0, s, Ar, Ai, ld, j0, n, cid, cvr, cvi, cnt,
-                               c->ipiv);
-            kt_close(c, kt, 0.0, 2.0 * sizeof(T) * (double)m * PB);
-            kt = kt_mark(c, KS_LASWP);
-            hipLaunchKernelGGL(k_laswp<T>, dim3((n + 255) / 256), dim3(256), 0, s, Ar, Ai, ld, n, j0, PB, c->ipiv);
-            kt_close(c, kt, 0.0, 8.0 * sizeof(T) * (double)n * PB);
-            const int nright = k0 + NB - (j0 + PB);
-            const int nblk = 1 + (m - PB + 255) / 256;
-            kt = kt_mark(c, KS_PANEL);
-            hipLaunchKernelGGL(k_panel<T>, dim3(nblk), dim3(256), 0, s, Ar, Ai, ld, j0, m, nright, dinfo);
-            kt_close(c, kt, 4.0 * (double)m * PB * PB, 4.0 * sizeof(T) * (double)m * PB);
-            if (nright > 0) {
-                const long o21 = (long)(j0 + PB) * ld + j0, o12 = (long)j0 * ld + j0 + PB;
-                const long o22 = (long)(j0 + PB) * ld + j0 + PB;
-                cgemm_ctx<T>(c, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22,
-                             Ai + o22, ld, 0);
-            }
+    for (int j0 = k0; j0 < k0 + NB; j0 += PB) {
+        const int m = n - j0;
+        const int nsets = (m + CH - 1) / CH;
+        int* pl = w.pl + ((j0 - k0) / PB) * PLS;
+        int kt = kt_mark(c, KS_TSLU, s);
+        hipLaunchKernelGGL(k_tslu<T>, dim3(nsets), dim3(256), 0, s, Ar, Ai, ld, j0, n, w.cid, w.cvr, w.cvi, w.cnt,
+                           c->ipiv, pl, w.d11r, w.d11i, w.info);
+        kt_close(c, kt, 0.0, 2.0 * sizeof(T) * (double)m * PB);
+        launch_laswp<T>(c, s, Ar, Ai, ld, k0, k0 + NB, 0, 0, pl, 1);
+        const int nright = k0 + NB - (j0 + PB);
+        const int nblk = 1 + (m - PB + 255) / 256;
+        kt = kt_mark(c, KS_PANEL, s);
+        hipLaunchKernelGGL(k_panel<T>, dim3(nblk), dim3(256), 0, s, Ar, Ai, ld, j0, m, nright, w.d11r, w.d11i);
+        kt_close(c, kt, 4.0 * (double)m * PB * PB, 4.0 * sizeof(T) * (double)m * PB);
+        if (nright > 0) {
+            const long o21 = (long)(j0 + PB) * ld + j0, o12 = (long)j0 * ld + j0 + PB;
+            const long o22 = (long)(j0 + PB) * ld + j0 + PB;
+            cgemm_ctx<T>(c, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22,
+                         ld, 0, s);
         }
-        // L11^{-1} of this diagonal block (kept for the solves)
+    }
+}
+
+// U12 = L11^{-1} A12 and A22 -= L21 U12 for the columns [c0, c0 + nc) right
+// of outer panel k0, on the context stream; U12 goes through the work buffer.
+template <typename T>
+void update_columns(bem_ctx* c, T* Ar, T* Ai, const T* Lir, const T* Lii, int k0, int c0, int nc,
+                    const LuWork<T>& w) {
+    const int n = c->npad;
+    const long ld = n;
+    const int nr = n - (k0 + NB);
+    const long io = (long)(k0 / NB) * NB * NB;
+    const long o12 = (long)k0 * ld + c0, o21 = (long)(k0 + NB) * ld + k0, o22 = (long)(k0 + NB) * ld + c0;
+    T* ubr = w.ubr + (c0 - (k0 + NB));
+    T* ubi = w.ubi + (c0 - (k0 + NB));
+    cgemm_ctx<T>(c, NB, nc, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1);
+    cgemm_ctx<T>(c, nr, nc, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0);
+    const int kt = kt_mark(c, KS_OTHER);
+    hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)(((long)NB * nc + 255) / 256)), dim3(256), 0, c->stream, NB,
+                       nc, ubr, ubi, ld, Ar + o12, Ai + o12, ld);
+    kt_close(c, kt, 0.0, 4.0 * sizeof(T) * (double)NB * nc);
+}
+
+// Blocked right-looking factorisation of the padded npad x npad matrix with
+// look-ahead of depth one: outer panel K+1 is factored on the panel stream
+// while the trailing update of panel K runs on the context stream.  Panel
+// K+1 only touches its own columns; its swaps reach the other columns on the
+// context stream once the update of panel K has finished.
+template <typename T>
+void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWork<T>& w) {
+    const int n = c->npad, nb = n / NB;
+    const long ld = n;
+    hipStream_t us = c->stream, ps = c->stream2;
+    hipEvent_t e_panel = c->lu_ev[0], e_next = c->lu_ev[1], e_start = c->lu_ev[2];
+    (void)hipEventRecord(e_start, us);  // the matrix and work buffers are ready
+    (void)hipStreamWaitEvent(ps, e_start, 0);
+    factor_panel<T>(c, ps, Ar, Ai, 0, w);
+    for (int K = 0; K < nb; K++) {
+        const int k0 = K * NB;
+        (void)hipEventRecord(e_panel, ps);
+        (void)hipStreamWaitEvent(us, e_panel, 0);
+        // swaps of panel K in every other column, then L11^{-1}
+        launch_laswp<T>(c, us, Ar, Ai, ld, 0, k0, k0 + NB, n, w.pl, NB / PB);
         const long o11 = (long)k0 * ld + k0;
-        const long io = (long)(k0 / NB) * NB * NB;
+        const long io = (long)K * NB * NB;
         int kt = kt_mark(c, KS_TRSM);
-        invert_diag_blocks<T>(s, NB, 1, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, true);
+        invert_diag_blocks<T>(us, NB, 1, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, true);
         kt_close(c, kt, 8.0 * NB * NB * NB / 6.0, 4.0 * sizeof(T) * (double)NB * NB);
         const int nr = n - (k0 + NB);
-        if (nr > 0) {
-            const long o12 = (long)k0 * ld + k0 + NB;
-            const long o21 = (long)(k0 + NB) * ld + k0, o22 = (long)(k0 + NB) * ld + k0 + NB;
-            // U12 = L11^{-1} A12 into the work buffer, trailing update, U12 back in place
-            cgemm_ctx<T>(c, NB, nr, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1);
-            cgemm_ctx<T>(c, nr, nr, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0);
-            kt = kt_mark(c, KS_OTHER);
-            hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)(((long)NB * nr + 255) / 256)), dim3(256), 0, s, NB,
-                               nr, ubr, ubi, ld, Ar + o12, Ai + o12, ld);
-            kt_close(c, kt, 0.0, 4.0 * sizeof(T) * (double)NB * nr);
-        }
+        if (nr <= 0) break;
+        // columns of panel K+1 first, then hand panel K+1 to the panel stream
+        update_columns<T>(c, Ar, Ai, Lir, Lii, k0, k0 + NB, NB, w);
+        (void)hipEventRecord(e_next, us);
+        (void)hipStreamWaitEvent(ps, e_next, 0);
+        factor_panel<T>(c, ps, Ar, Ai, k0 + NB, w);
+        if (nr > NB) update_columns<T>(c, Ar, Ai, Lir, Lii, k0, k0 + 2 * NB, nr - NB, w);
     }
     // U^{-1} of all diagonal blocks in one launch
     const int kt = kt_mark(c, KS_TRSM);
-    invert_diag_blocks<T>(s, NB, n / NB, Ar, Ai, ld, Uir, Uii, false);
+    invert_diag_blocks<T>(us, NB, nb, Ar, Ai, ld, Uir, Uii, false);
     kt_close(c, kt, 8.0 * (double)n * NB * NB / 6.0, 4.0 * sizeof(T) * (double)n * NB);
 }
 
@@ -417,23 +563,30 @@ int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     T* Lii = Lir + (size_t)n * NB;
     T* Uir = Lii + (size_t)n * NB;
     T* Uii = Uir + (size_t)n * NB;
-    // work: info + counters, candidate ids, candidate values, U12 buffer
+    // work: info, counters, row-pair list, candidate ids, candidate values,
+    // diagonal block, U12 buffer
     const int nsets0 = (n + CH - 1) / CH;
     const size_t ncnt = 2 * (size_t)nsets0 + 64;
-    const size_t bytes = 256 + ncnt * sizeof(int) + 2 * (size_t)nsets0 * PB * sizeof(int) +
-                         4 * (size_t)nsets0 * PB * PB * sizeof(T) + 2 * (size_t)NB * n * sizeof(T) + 256;
+    const size_t nint = 64 + ncnt + (NB / PB) * PLS + 2 * (size_t)nsets0 * PB;
+    const size_t bytes = nint * sizeof(int) + 256 + 4 * (size_t)nsets0 * PB * PB * sizeof(T) +
+                         2 * (size_t)PB * PB * sizeof(T) + 2 * (size_t)NB * n * sizeof(T) + 256;
     void* scr = nullptr;
     if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
-    char* p = static_cast<char*>(scr);
-    int* dinfo = reinterpret_cast<int*>(p);
-    int* cnt = dinfo + 64;
-    int* cid = cnt + ncnt;
-    T* cvr = reinterpret_cast<T*>(cid + 2 * (size_t)nsets0 * PB);
-    T* cvi = cvr + 2 * (size_t)nsets0 * PB * PB;
-    T* ubr = cvi + 2 * (size_t)nsets0 * PB * PB;
-    T* ubi = ubr + (size_t)NB * n;
-    BEM_HIP(c, hipMemsetAsync(dinfo, 0, (64 + ncnt) * sizeof(int), c->stream));
-    factor<T>(c, Ar, Ai, Lir, Lii, Uir, Uii, dinfo, cid, cvr, cvi, cnt, ubr, ubi);
+    LuWork<T> w;
+    w.info = static_cast<int*>(scr);
+    w.cnt = w.info + 64;
+    w.pl = w.cnt + ncnt;
+    w.cid = w.pl + (NB / PB) * PLS;
+    char* p = reinterpret_cast<char*>(w.info + nint);
+    p += (256 - reinterpret_cast<uintptr_t>(p) % 256) % 256;
+    w.cvr = reinterpret_cast<T*>(p);
+    w.cvi = w.cvr + 2 * (size_t)nsets0 * PB * PB;
+    w.d11r = w.cvi + 2 * (size_t)nsets0 * PB * PB;
+    w.d11i = w.d11r + PB * PB;
+    w.ubr = w.d11i + PB * PB;
+    w.ubi = w.ubr + (size_t)NB * n;
+    BEM_HIP(c, hipMemsetAsync(w.info, 0, (64 + ncnt) * sizeof(int), c->stream));
+    factor<T>(c, Ar, Ai, Lir, Lii, Uir, Uii, w);
     return 0;
 }
 

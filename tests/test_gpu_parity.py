@@ -89,7 +89,7 @@ def test_LK_vs_oracle(oracle, ctx):
 
 
 # ---------------------------------------------------------------- LU
-@pytest.mark.parametrize("n,nrhs", [(40, 1), (300, 5), (700, 17), (1024, 3)])
+@pytest.mark.parametrize("n,nrhs", [(40, 1), (300, 5), (700, 17), (1024, 3), (2500, 2)])
 def test_lu_fp64_vs_numpy(ctx, n, nrhs):
     rs = np.random.RandomState(42)
     Z = rs.randn(n, n) + 1j * rs.randn(n, n)
@@ -119,13 +119,35 @@ def test_lu_fp32_refined_vs_numpy(ctx, n, nrhs):
     assert np.abs(X - Xr).max() <= 1e-10 * np.abs(Xr).max()
 
 
-def test_lu_singular_info(ctx):
-    rs = np.random.RandomState(3)
-    Z = rs.randn(64, 64) + 0j
-    Z[:, 10] = 0.0
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lu_zero_diagonal_needs_pivoting(ctx, prec):
+    # every leading minor of order 1 is zero: no pivot-free elimination exists
+    rs = np.random.RandomState(11)
+    n = 600
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n)
+    Z[np.arange(n), np.arange(n)] = 0.0
+    B = rs.randn(n, 3) + 1j * rs.randn(n, 3)
     ctx.set_Z(Z)
-    info = ctx.factor("fp64")
-    assert info > 0
+    assert ctx.factor(prec) == 0
+    X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
+    be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
+    assert be < 2e-15
+    # the pivots form a permutation
+    ip = ctx.get_pivots(n) - 1
+    perm = np.arange(n)
+    for i in range(n):
+        perm[[i, ip[i]]] = perm[[ip[i], i]]
+    assert sorted(perm.tolist()) == list(range(n))
+
+
+@pytest.mark.parametrize("n,col", [(64, 10), (700, 300), (700, 699)])
+def test_lu_singular_info(ctx, n, col):
+    # a zero column gives the first zero pivot exactly there (LAPACK info = col + 1)
+    rs = np.random.RandomState(3)
+    Z = rs.randn(n, n) + 0j
+    Z[:, col] = 0.0
+    ctx.set_Z(Z)
+    assert ctx.factor("fp64") == col + 1
 
 
 def test_lu_get_set_roundtrip(ctx):
