@@ -622,6 +622,7 @@ int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, i
         BEM_HIP(c, hipMemcpy(c->Br, re.data(), re.size() * sizeof(double), hipMemcpyHostToDevice));
         BEM_HIP(c, hipMemcpy(c->Bi, im.data(), im.size() * sizeof(double), hipMemcpyHostToDevice));
         c->has_B = true;
+        c->rhs_planewave = false;
     } else {
         if (!c->has_B) return bem_fail(c, BEM_ERR_STATE, "solve: no right-hand sides on the device");
         nrhs = c->nrhs;
@@ -667,6 +668,7 @@ int bem_rhs_planewave(bem_ctx* c, double kr, double ki, double eta, const double
     if (int rc = rhs_launch(c, kr, ki, eta, khat, E0, nrhs)) return rc;
     c->t_ms[5] = tm.stop();
     c->has_B = true;
+    c->rhs_planewave = true;
     if (B_out) {
         const int n = c->n, ld = c->ldr;
         std::vector<double> re((size_t)n * ld), im((size_t)n * ld);

@@ -32,6 +32,20 @@ struct AsmParams {
     double eta[2];        // eta_ext, eta_int
 };
 
+// Half-pair records as general combinations of the per-wavenumber parts
+// U_w = c_a c_b lm_w (+ self static part), V_w = c_a c_b m0_w (+ static),
+// K_w = c_a c_b km_w (forward: L_w = i k_w U_w - (4i/k_w) V_w):
+//   P = sum_w aP_w U_w + bP_w V_w,  Q = sum_w aQ_w U_w + bQ_w V_w,
+//   Ks = sum_w cK_w K_w,
+// complex coefficients [w][re, im].  `deriv` selects the moments of dG/dk and
+// dg/dk instead of G and g (no static part; r0[w] is dG/dk at R = 0 of a self
+// pair, given the clamped R = 0 value i Re(k)/(4 pi) of the forward kernel).
+struct RecCoef {
+    double aP[2][2], bP[2][2], aQ[2][2], bQ[2][2], cK[2][2];
+    double r0[2][2];
+    int deriv;
+};
+
 // Per-kernel-class device timing (bem_kernel_stats): HIP events around each
 // launch on the context stream, resolved when the ABI call returns.
 enum KernelClass {
@@ -109,6 +123,11 @@ struct bem_ctx {
     size_t rhs_cap = 0;  // doubles per plane allocated
     bool has_B = false, has_X = false;
 
+    // plane waves of the last bem_rhs_planewave call (k_hat, E0, H0 per RHS)
+    std::vector<double> rhs_prm;
+    double rhs_k[2] = {0, 0}, rhs_eta = 1.0;
+    bool rhs_planewave = false;
+
     // far field / Mueller of the last bem_orient_mueller call (for the adjoint)
     std::vector<double> last_RT, last_w, last_theta;
     int last_norient = 0, last_ntheta = 0, last_mirror = 0;
@@ -147,6 +166,17 @@ int ff_launch(bem_ctx* c, const double* dJr, const double* dJi, const double* dM
 int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* RT, const double* w, int n_orient,
                    const double* theta, int ntheta, int mirror, double* M_avg, double* M_orient);
 int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad);
+// d B / d k_ext of the stored plane waves into (dBr, dBi) (npad x ldr).
+int rhs_dk_launch(bem_ctx* c, double* dBr, double* dBi);
+// Far-field part of the adjoint for the last bem_orient_mueller call: given
+// dL/dM_avg (host, 16 x ntheta), writes g = dL/dX (holomorphic derivative,
+// npad x ldr planar, device) and the explicit dL/dk_ext at fixed X.
+int ff_adjoint_launch(bem_ctx* c, const double* dL_dM, double* gr, double* gi, double* explicit_dk);
+// <Z_q, W> = sum_{m,n} Z_q[m][n] W[m][n] over the 2N x 2N matrix whose
+// half-pair records are combined with q (PMCHWT block pattern of Z), W planar
+// FP64 row-major with leading dimension ldw; result (re, im) in out[2].
+int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef& q, const double* Wr, const double* Wi,
+                        long ldw, double* out);
 
 // Small helpers used by several launchers
 int ctx_ensure_rhs(bem_ctx* c, int nrhs);

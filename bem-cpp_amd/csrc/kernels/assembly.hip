@@ -154,13 +154,13 @@ __device__ __forceinline__ void mom_fold(Mom& m, const Inner& s, const double4& 
     }
 }
 
-// L_ab and K_ab from the moments (formulas in the file header).
-__device__ __forceinline__ void half_pair(const Mom& m, double kr, double ki, double ca, const double* pa, double cb,
-                                          const double* pb, C2& L, C2& K) {
+// lm = Mrs - p_a.Ms - p_b.Mr + (p_a.p_b) M0 and km = (p_b - p_a).Nx +
+// (p_b x p_a).Nd of one half pair (formulas in the file header).
+__device__ __forceinline__ void raw_parts(const Mom& m, const double* pa, const double* pb, double lm[2],
+                                          double km[2]) {
     const double papb = pa[0] * pb[0] + pa[1] * pb[1] + pa[2] * pb[2];
     const double cr[3] = {pb[1] * pa[2] - pb[2] * pa[1], pb[2] * pa[0] - pb[0] * pa[2],
                           pb[0] * pa[1] - pb[1] * pa[0]};  // p_b x p_a
-    double lm[2], km[2];
 #pragma unroll
     for (int c = 0; c < 2; c++) {
         double v = m.mrs[c] + papb * m.m0[c];
@@ -173,6 +173,13 @@ __device__ __forceinline__ void half_pair(const Mom& m, double kr, double ki, do
         lm[c] = v;
         km[c] = k;
     }
+}
+
+// L_ab and K_ab from the moments (formulas in the file header).
+__device__ __forceinline__ void half_pair(const Mom& m, double kr, double ki, double ca, const double* pa, double cb,
+                                          const double* pb, C2& L, C2& K) {
+    double lm[2], km[2];
+    raw_parts(m, pa, pb, lm, km);
     // ik * lm - (4i/k) m0, with i/k = (ki + i kr)/|k|^2
     const double k2 = kr * kr + ki * ki;
     const double fr = 4.0 * ki / k2, fi = 4.0 * kr / k2;
@@ -216,6 +223,52 @@ __device__ __forceinline__ void write_record(double* __restrict__ H, long long r
     }
 }
 
+// acc += (a[0] + i a[1]) (x + i y)
+__device__ __forceinline__ void cmac(double acc[2], const double a[2], double x, double y) {
+    acc[0] = fma(a[0], x, fma(-a[1], y, acc[0]));
+    acc[1] = fma(a[0], y, fma(a[1], x, acc[1]));
+}
+
+// Writes the 9 half-pair values of pair (t, s) combined with q (RecCoef);
+// `sv` (self pairs, moments of G only) holds the static parts vec_ab (9
+// values) and S_P (sv[9]), which enter U and V with c_a c_b / (4 pi).
+__device__ __forceinline__ void write_record_coef(double* __restrict__ H, long long rec, const Mom* mom,
+                                                  const RecCoef& q, const double* __restrict__ hc,
+                                                  const double* __restrict__ hp, int t, int s, const double* sv) {
+    double* out = H + rec * kRec;
+#pragma unroll 1
+    for (int a = 0; a < 3; a++) {
+        const double ca = hc[t * 3 + a];
+        const double pa[3] = {hp[(t * 3 + a) * 3], hp[(t * 3 + a) * 3 + 1], hp[(t * 3 + a) * 3 + 2]};
+#pragma unroll 1
+        for (int b = 0; b < 3; b++) {
+            const double cb = hc[s * 3 + b];
+            const double pb[3] = {hp[(s * 3 + b) * 3], hp[(s * 3 + b) * 3 + 1], hp[(s * 3 + b) * 3 + 2]};
+            const double cc = ca * cb;
+            double P[2] = {0.0, 0.0}, Q[2] = {0.0, 0.0}, K[2] = {0.0, 0.0};
+#pragma unroll
+            for (int w = 0; w < 2; w++) {
+                double lm[2], km[2];
+                raw_parts(mom[w], pa, pb, lm, km);
+                double U[2] = {cc * lm[0], cc * lm[1]}, V[2] = {cc * mom[w].m0[0], cc * mom[w].m0[1]};
+                if (sv) {
+                    U[0] += cc * kInv4Pi * sv[a * 3 + b];
+                    V[0] += cc * kInv4Pi * sv[9];
+                }
+                cmac(P, q.aP[w], U[0], U[1]);
+                cmac(P, q.bP[w], V[0], V[1]);
+                cmac(Q, q.aQ[w], U[0], U[1]);
+                cmac(Q, q.bQ[w], V[0], V[1]);
+                cmac(K, q.cK[w], cc * km[0], cc * km[1]);
+            }
+            double2* o = reinterpret_cast<double2*>(out + (a * 3 + b) * 6);
+            o[0] = make_double2(P[0], P[1]);
+            o[1] = make_double2(Q[0], Q[1]);
+            o[2] = make_double2(K[0], K[1]);
+        }
+    }
+}
+
 __device__ __forceinline__ bool touching(const int4& a, const int4& b) {
     return a.x == b.x || a.x == b.y || a.x == b.z || a.y == b.x || a.y == b.y || a.y == b.z || a.z == b.x ||
            a.z == b.y || a.z == b.z;
@@ -225,10 +278,12 @@ __device__ __forceinline__ bool touching(const int4& a, const int4& b) {
 // Non-touching triangle pairs.  Block = 16 x 16 triangles (t in the row tile,
 // s in the column tile), only tiles with row <= column; thread (ti, tj).
 // ---------------------------------------------------------------------------
-template <int NQ, bool LOSSY>
+// MODE 0: forward records; 1: RecCoef records of the moments of G, g;
+// 2: RecCoef records of the moments of dG/dk = iR G and dg/dk = -k G.
+template <int NQ, bool LOSSY, int MODE>
 __global__ __launch_bounds__(256) void k_tripair_far(int T, const double4* __restrict__ qp,
                                                      const int4* __restrict__ tv, const double* __restrict__ hc,
-                                                     const double* __restrict__ hp, AsmParams p,
+                                                     const double* __restrict__ hp, AsmParams p, RecCoef q,
                                                      double* __restrict__ H) {
     const int bt = blockIdx.y, bs = blockIdx.x;
     if (bt > bs) return;
@@ -278,22 +333,35 @@ __global__ __launch_bounds__(256) void k_tripair_far(int T, const double4* __res
                 sincos_quarter(R * kq0, sn, cs);
                 const float a0 = lossy0 ? amp * exp_neg(-p.ki[0] * R) : amp;
                 const float Gr = a0 * cs, Gi = a0 * sn;
-                const float ar = (-kif0 - invR) * invR, ai = krf0 * invR;
-                inner_add(in0, Q.w, Gr, Gi, Gr * ar - Gi * ai, Gr * ai + Gi * ar, Q);
+                if (MODE == 2) {
+                    const float Rf = (float)R;
+                    inner_add(in0, Q.w, -Rf * Gi, Rf * Gr, -(krf0 * Gr - kif0 * Gi), -(krf0 * Gi + kif0 * Gr), Q);
+                } else {
+                    const float ar = (-kif0 - invR) * invR, ai = krf0 * invR;
+                    inner_add(in0, Q.w, Gr, Gi, Gr * ar - Gi * ai, Gr * ai + Gi * ar, Q);
+                }
             }
             {  // interior wavenumber
                 float sn, cs;
                 sincos_quarter(R * kq1, sn, cs);
                 const float a1 = LOSSY ? amp * exp_neg(-p.ki[1] * R) : amp;
                 const float Gr = a1 * cs, Gi = a1 * sn;
-                const float ar = (-kif1 - invR) * invR, ai = krf1 * invR;
-                inner_add(in1, Q.w, Gr, Gi, Gr * ar - Gi * ai, Gr * ai + Gi * ar, Q);
+                if (MODE == 2) {
+                    const float Rf = (float)R;
+                    inner_add(in1, Q.w, -Rf * Gi, Rf * Gr, -(krf1 * Gr - kif1 * Gi), -(krf1 * Gi + kif1 * Gr), Q);
+                } else {
+                    const float ar = (-kif1 - invR) * invR, ai = krf1 * invR;
+                    inner_add(in1, Q.w, Gr, Gi, Gr * ar - Gi * ai, Gr * ai + Gi * ar, Q);
+                }
             }
         }
         mom_fold(mom[0], in0, P);
         mom_fold(mom[1], in1, P);
     }
-    write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+    if (MODE == 0)
+        write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+    else
+        write_record_coef(H, rec_index(t, s, T), mom, q, hc, hp, t, s, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -391,12 +459,12 @@ __global__ __launch_bounds__(256) void k_graglia(int T, const double4* __restric
 //                                                 + (p_a.p_b) S_P) ]
 // with S_P = sum jw P, S_Pr = sum jw P r, S_V = sum jw V, S_rV = sum jw r.V.
 // ---------------------------------------------------------------------------
-template <int NQ>
+template <int NQ, int MODE>
 __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const int2* __restrict__ pairs,
                                                       const double4* __restrict__ qp,
                                                       const double* __restrict__ hc,
                                                       const double* __restrict__ hp,
-                                                      const double4* __restrict__ pv, AsmParams p,
+                                                      const double4* __restrict__ pv, AsmParams p, RecCoef q,
                                                       double* __restrict__ H) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_near) return;
@@ -422,20 +490,36 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
                 const double kr = p.kr[kk], ki = p.ki[kk];
                 double Gr, Gi, gr = 0.0, gi = 0.0;
                 if (self && R < 1e-12) {
-                    Gr = 0.0;
-                    Gi = kr * kInv4Pi;
+                    if (MODE == 2) {
+                        Gr = q.r0[kk][0];
+                        Gi = q.r0[kk][1];
+                    } else {
+                        Gr = 0.0;
+                        Gi = kr * kInv4Pi;
+                    }
                 } else {
                     double sn, cs;
                     sincos(kr * R, &sn, &cs);
                     const double amp = exp(-ki * R) * kInv4Pi / R;
                     Gr = amp * cs;
                     Gi = amp * sn;
-                    if (!self && R > 1e-6) {
-                        const double ar = (-ki - 1.0 / R) / R, ai = kr / R;
-                        gr = Gr * ar - Gi * ai;
-                        gi = Gr * ai + Gi * ar;
+                    const bool with_g = !self && R > 1e-6;
+                    if (MODE == 2) {  // dG/dk = iR G (the static 1/(4 pi R) has none), dg/dk = -k G
+                        if (with_g) {
+                            gr = -(kr * Gr - ki * Gi);
+                            gi = -(kr * Gi + ki * Gr);
+                        }
+                        const double t0 = -R * Gi;
+                        Gi = R * Gr;
+                        Gr = t0;
+                    } else {
+                        if (with_g) {
+                            const double ar = (-ki - 1.0 / R) / R, ai = kr / R;
+                            gr = Gr * ar - Gi * ai;
+                            gi = Gr * ai + Gi * ar;
+                        }
+                        if (self) Gr -= kInv4Pi / R;
                     }
-                    if (self) Gr -= kInv4Pi / R;
                 }
                 inner_add(in[kk], Q.w, Gr, Gi, gr, gi, Q);
             }
@@ -443,8 +527,11 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
         mom_fold(mom[0], in[0], P);
         mom_fold(mom[1], in[1], P);
     }
-    if (!self) {
-        write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+    if (!self || MODE == 2) {
+        if (MODE == 0)
+            write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+        else
+            write_record_coef(H, rec_index(t, s, T), mom, q, hc, hp, t, s, nullptr);
         return;
     }
     double SP = 0.0, SPr[3] = {0, 0, 0}, SV[3] = {0, 0, 0}, SrV = 0.0;
@@ -461,6 +548,8 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
         SrV += x.w * (x.x * w.y + x.y * w.z + x.z * w.w);
     }
     C2 corr[18];
+    double sv[10];
+    sv[9] = SP;
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) {
             const double ca = hc[t * 3 + a], cb = hc[t * 3 + b];
@@ -472,6 +561,7 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
                 papb += pa[e] * pb[e];
             }
             const double vec = SrV - 0.5 * sym + papb * SP;  // real
+            sv[a * 3 + b] = vec;
             for (int kk = 0; kk < 2; kk++) {
                 const double kr = p.kr[kk], ki = p.ki[kk], k2 = kr * kr + ki * ki;
                 // -(4i/k) SP + ik vec, i/k = (ki + i kr)/|k|^2, ik = -ki + i kr
@@ -482,7 +572,10 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
                 corr[kk * 9 + a * 3 + b].i = f * im;
             }
         }
-    write_record(H, rec_index(t, t, T), mom, p, hc, hp, t, t, corr);
+    if (MODE == 0)
+        write_record(H, rec_index(t, t, T), mom, p, hc, hp, t, t, corr);
+    else
+        write_record_coef(H, rec_index(t, t, T), mom, q, hc, hp, t, t, sv);
 }
 
 // ---------------------------------------------------------------------------
@@ -547,30 +640,123 @@ __global__ void k_pad(int n, int ld, F* __restrict__ Ar, F* __restrict__ Ai) {
     Ai[(size_t)i * ld + j] = F(0);
 }
 
+// <Z_H, W> over the 2N x 2N matrix whose entries k_gather would form from the
+// records: thread (m, n) sums its four PMCHWT entries times W, the block sums
+// in a fixed tree order into part[(m * gridDim.x + blockIdx.x) * 2 + {0, 1}].
+__global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __restrict__ rwg_half,
+                                                  const double* __restrict__ H, const double* __restrict__ Wr,
+                                                  const double* __restrict__ Wi, long ldw, double* __restrict__ part) {
+    __shared__ double red[2][256];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    double acc[2] = {0.0, 0.0};
+    if (n < N) {
+        const int4 hm = rwg_half[m], hn = rwg_half[n];
+        const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
+        const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
+        double P[2] = {0, 0}, Q[2] = {0, 0}, K[2] = {0, 0};
+#pragma unroll
+        for (int sa = 0; sa < 2; sa++)
+#pragma unroll
+            for (int sb = 0; sb < 2; sb++) {
+                const int ta = tm[sa], tb = tn[sb];
+                long long rec;
+                int slot;
+                if (ta <= tb) {
+                    rec = rec_index(ta, tb, T);
+                    slot = lm[sa] * 3 + ln[sb];
+                } else {
+                    rec = rec_index(tb, ta, T);
+                    slot = ln[sb] * 3 + lm[sa];
+                }
+                const double2* h = reinterpret_cast<const double2*>(H + rec * kRec + slot * 6);
+                const double2 a = h[0], b = h[1], c = h[2];
+                P[0] += a.x; P[1] += a.y;
+                Q[0] += b.x; Q[1] += b.y;
+                K[0] += c.x; K[1] += c.y;
+            }
+        const size_t r0 = (size_t)m * ldw, r1 = (size_t)(N + m) * ldw;
+        const double mK[2] = {-K[0], -K[1]};
+        cmac(acc, P, Wr[r0 + n], Wi[r0 + n]);
+        cmac(acc, mK, Wr[r0 + N + n], Wi[r0 + N + n]);
+        cmac(acc, K, Wr[r1 + n], Wi[r1 + n]);
+        cmac(acc, Q, Wr[r1 + N + n], Wi[r1 + N + n]);
+    }
+    red[0][threadIdx.x] = acc[0];
+    red[1][threadIdx.x] = acc[1];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const long o = ((long)m * gridDim.x + blockIdx.x) * 2;
+        part[o] = red[0][0];
+        part[o + 1] = red[1][0];
+    }
+}
+
+// out[0..1] = sum of the np complex partials, one block, fixed order.
+__global__ __launch_bounds__(256) void k_sum_parts(long np, const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double red[2][256];
+    double a = 0.0, b = 0.0;
+    for (long i = threadIdx.x; i < np; i += 256) {
+        a += part[2 * i];
+        b += part[2 * i + 1];
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0][0];
+        out[1] = red[1][0];
+    }
+}
+
+// Graglia data (modes 0 and 1) and the records of all triangle pairs into c->H.
+template <int NQ, int MODE>
+void launch_records(bem_ctx* c, const AsmParams& p, const RecCoef& q, bool lossy, const double* dverts,
+                    const double* dqw) {
+    const DevMesh& d = c->dm;
+    const int T = d.T;
+    if (MODE != 2)
+        hipLaunchKernelGGL(k_graglia<NQ>, dim3((T * NQ + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv,
+                           dverts, dqw, d.pv);
+    const int nb = (T + 15) / 16;
+    const double npair = (double)T * (T + 1) / 2;
+    int kt = kt_mark(c, KS_TRIPAIR_FAR);
+    if (lossy)
+        hipLaunchKernelGGL((k_tripair_far<NQ, true, MODE>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv,
+                           d.hc, d.hp, p, q, c->H);
+    else
+        hipLaunchKernelGGL((k_tripair_far<NQ, false, MODE>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv,
+                           d.hc, d.hp, p, q, c->H);
+    kt_close(c, kt, npair * NQ * NQ, npair * kRec * 8.0);  // unit: triangle point pairs
+    kt = kt_mark(c, KS_TRIPAIR_NEAR);
+    if (d.n_near > 0)
+        hipLaunchKernelGGL((k_tripair_near<NQ, MODE>), dim3((d.n_near + 127) / 128), dim3(128), 0, c->stream, T,
+                           d.n_near, d.near, d.qp, d.hc, d.hp, d.pv, p, q, c->H);
+    kt_close(c, kt, (double)d.n_near * NQ * NQ, (double)d.n_near * kRec * 8.0);
+}
+
 template <int NQ>
 int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi, float* Wr, float* Wi,
               const double* dverts, const double* dqw) {
     const DevMesh& d = c->dm;
     const int T = d.T;
-    hipLaunchKernelGGL(k_graglia<NQ>, dim3((T * NQ + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv, dverts,
-                       dqw, d.pv);
-    const int nb = (T + 15) / 16;
-    const double npair = (double)T * (T + 1) / 2;
-    int kt = kt_mark(c, KS_TRIPAIR_FAR);
-    if (lossy)
-        hipLaunchKernelGGL((k_tripair_far<NQ, true>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv, d.hc,
-                           d.hp, p, c->H);
-    else
-        hipLaunchKernelGGL((k_tripair_far<NQ, false>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv,
-                           d.hc, d.hp, p, c->H);
-    kt_close(c, kt, npair * NQ * NQ, npair * kRec * 8.0);  // unit: triangle point pairs
-    kt = kt_mark(c, KS_TRIPAIR_NEAR);
-    if (d.n_near > 0)
-        hipLaunchKernelGGL(k_tripair_near<NQ>, dim3((d.n_near + 127) / 128), dim3(128), 0, c->stream, T, d.n_near,
-                           d.near, d.qp, d.hc, d.hp, d.pv, p, c->H);
-    kt_close(c, kt, (double)d.n_near * NQ * NQ, (double)d.n_near * kRec * 8.0);
+    launch_records<NQ, 0>(c, p, RecCoef{}, lossy, dverts, dqw);
     const int N = d.N;
-    kt = kt_mark(c, KS_GATHER);
+    const int kt = kt_mark(c, KS_GATHER);
     hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256, N), dim3(256), 0, c->stream, N, T, c->npad, d.rwg_half, c->H,
                        Zr, Zi, Wr, Wi);
     kt_close(c, kt, 0.0, (double)N * N * (4.0 * 48.0 + (Zr ? 64.0 : 0.0) + (Wr ? 32.0 : 0.0)));
@@ -588,9 +774,9 @@ int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi
 
 }  // namespace
 
-// Device copies of the vertex coordinates and quadrature weights used by
-// k_graglia live in the context scratch area.
-int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
+// Record buffer of T(T+1)/2 records, and device copies of the vertex
+// coordinates and quadrature weights used by k_graglia (context scratch).
+static int asm_prepare(bem_ctx* c, const double** dverts, const double** dqw) {
     const DevMesh& d = c->dm;
     const size_t T = (size_t)d.T;
     const size_t need = T * (T + 1) / 2 * kRec * sizeof(double);
@@ -604,7 +790,6 @@ int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
         }
         c->H_bytes = need;
     }
-    // vertex coordinates + weights for k_graglia
     const int nv = c->mesh.nv();
     const size_t sbytes = (size_t)(nv * 3 + 32) * sizeof(double);
     void* s = nullptr;
@@ -616,23 +801,60 @@ int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
         host[3 * i + 2] = c->mesh.verts[i].z;
     }
     for (int q = 0; q < c->quad.npts; q++) host[nv * 3 + q] = c->quad.wts[q];
-    BEM_HIP(c, hipMemcpyAsync(s, host.data(), sbytes, hipMemcpyHostToDevice, c->stream));
-    const double* dverts = static_cast<const double*>(s);
-    const double* dqw = dverts + nv * 3;
+    BEM_HIP(c, hipMemcpy(s, host.data(), sbytes, hipMemcpyHostToDevice));
+    *dverts = static_cast<const double*>(s);
+    *dqw = *dverts + nv * 3;
+    return 0;
+}
+
+int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
+    const double *dverts, *dqw;
+    if (int rc = asm_prepare(c, &dverts, &dqw)) return rc;
     const bool lossy = p.ki[1] != 0.0;
     double* Zr = want64 ? c->Zr : nullptr;
     double* Zi = want64 ? c->Zi : nullptr;
     float* Wr = want32 ? c->Wr : nullptr;
     float* Wi = want32 ? c->Wi : nullptr;
     int rc;
-    switch (d.Nq) {
+    switch (c->dm.Nq) {
         case 4: rc = launch_nq<4>(c, p, lossy, Zr, Zi, Wr, Wi, dverts, dqw); break;
         case 13: rc = launch_nq<13>(c, p, lossy, Zr, Zi, Wr, Wi, dverts, dqw); break;
         default: rc = launch_nq<7>(c, p, lossy, Zr, Zi, Wr, Wi, dverts, dqw); break;
     }
     if (rc) return rc;
     BEM_HIP(c, hipGetLastError());
-    // the host staging vector must outlive the copy
     BEM_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef& q, const double* Wr, const double* Wi,
+                        long ldw, double* out) {
+    const double *dverts, *dqw;
+    if (int rc = asm_prepare(c, &dverts, &dqw)) return rc;
+    const bool lossy = p.ki[1] != 0.0;
+    const int Nq = c->dm.Nq;
+    if (q.deriv) {
+        if (Nq == 4) launch_records<4, 2>(c, p, q, lossy, dverts, dqw);
+        else if (Nq == 13) launch_records<13, 2>(c, p, q, lossy, dverts, dqw);
+        else launch_records<7, 2>(c, p, q, lossy, dverts, dqw);
+    } else {
+        if (Nq == 4) launch_records<4, 1>(c, p, q, lossy, dverts, dqw);
+        else if (Nq == 13) launch_records<13, 1>(c, p, q, lossy, dverts, dqw);
+        else launch_records<7, 1>(c, p, q, lossy, dverts, dqw);
+    }
+    const int N = c->dm.N;
+    const int gx = (N + 255) / 256;
+    const long np = (long)gx * N;
+    double* part = nullptr;
+    if (hipMalloc(&part, (np * 2 + 2) * sizeof(double)) != hipSuccess)
+        return bem_fail(c, BEM_ERR_NOMEM, "contraction partial sums");
+    hipLaunchKernelGGL(k_contract, dim3(gx, N), dim3(256), 0, c->stream, N, c->dm.T, c->dm.rwg_half, c->H, Wr, Wi,
+                       ldw, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, np, part, part + np * 2);
+    const hipError_t e1 = hipMemcpyAsync(out, part + np * 2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(part);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return bem_fail(c, BEM_ERR_HIP, std::string("contraction: ") + hipGetErrorString(e1 ? e1 : e2));
     return 0;
 }

@@ -28,10 +28,11 @@ constexpr int kTile = 64;     // points per LDS tile
 // RHS: B[n][j] = sum_halves sum_q f.E0_j e^{ik k_j.r} jw, B[N+n][j] likewise with
 // H0_j = k_j x E0_j / eta.  prm: per RHS (k_hat, E0, H0) = 9 doubles.
 // ---------------------------------------------------------------------------
+// dk = 1: the derivative with respect to k (phase factor i k_hat.r).
 __global__ __launch_bounds__(256) void k_rhs(int N, int Nq, int npad, int ldr, int nrhs,
                                              const double4* __restrict__ qp, const double* __restrict__ hc,
                                              const double* __restrict__ hp, const int4* __restrict__ rwg_half,
-                                             const double* __restrict__ prm, double kr, double ki,
+                                             const double* __restrict__ prm, double kr, double ki, int dk,
                                              double* __restrict__ Br, double* __restrict__ Bi) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)npad * ldr) return;
@@ -64,7 +65,12 @@ __global__ __launch_bounds__(256) void k_rhs(int N, int Nq, int npad, int ldr, i
             double sn, cs;
             sincos(kr * sd, &sn, &cs);
             const double ea = exp(-ki * sd);
-            const double pr = ea * cs * x.w, pi = ea * sn * x.w;
+            double pr = ea * cs * x.w, pi = ea * sn * x.w;
+            if (dk) {
+                const double t0 = -sd * pi;
+                pi = sd * pr;
+                pr = t0;
+            }
             const double fe = fx * P[3] + fy * P[4] + fz * P[5];
             const double fh = fx * P[6] + fy * P[7] + fz * P[8];
             sEr += fe * pr; sEi += fe * pi;
@@ -316,6 +322,251 @@ int farfield_core(bem_ctx* c, const double* Jr, const double* Ji, const double* 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the far-field chain (holomorphic derivatives: for the real loss
+// L, dL = 2 Re sum (dL/dv) dv).  Per (orientation o, angle t), given
+// G = dL/dM_avg: sigma_l = dL/dS_l from the Mueller quadratic forms with
+// weights G_e w_o / |k|^2, then dL/dF (S = -ik F.e), dL/dV (F = -ik/(4 pi) V)
+// and coef = (dL/dJ~, dL/dM~) of both polarisations with
+// V = eta (J~ - rhat (rhat.J~)) - rhat x M~.  expl[o][t] receives the explicit
+// dL/dk of this (o, t) through the prefactors: 2 Re sum sigma_l (2/k) S_l
+// (S = -k^2/(4 pi) V.e) and -(2/k) sum G_e w_o M_e / |k|^2 (the 1/|k|^2
+// normalisation; k = k_ext real).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_mueller_bwd(int n_orient, int ntheta, const double* __restrict__ Fv,
+                                                     const double* __restrict__ geo, const double* __restrict__ G,
+                                                     const double* __restrict__ w, double kr, double ki, double eta,
+                                                     int mirror, double* __restrict__ coef,
+                                                     double* __restrict__ expl) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_orient * ntheta) return;
+    const int o = idx / ntheta, t = idx % ntheta;
+    const double* g = geo + (long)idx * 9;
+    const double* F2[2] = {Fv + ((long)(2 * o) * ntheta + t) * 6, Fv + ((long)(2 * o + 1) * ntheta + t) * 6};
+    auto proj = [](const double* F, const double* e, double* out) {
+        out[0] = F[0] * e[0] + F[2] * e[1] + F[4] * e[2];
+        out[1] = F[1] * e[0] + F[3] * e[1] + F[5] * e[2];
+    };
+    double a[4][2];  // F_par.ep, F_par.epp, F_perp.ep, F_perp.epp
+    proj(F2[0], g + 3, a[0]);
+    proj(F2[0], g + 6, a[1]);
+    proj(F2[1], g + 3, a[2]);
+    proj(F2[1], g + 6, a[3]);
+    const double mik[2] = {ki, -kr};  // -ik
+    double S[5][2];
+    const int map[4] = {2, 4, 3, 1};
+    for (int u = 0; u < 4; u++) {
+        S[map[u]][0] = mik[0] * a[u][0] - mik[1] * a[u][1];
+        S[map[u]][1] = mik[0] * a[u][1] + mik[1] * a[u][0];
+    }
+    const double k2 = kr * kr + ki * ki;
+    double cw[16];
+    for (int e = 0; e < 16; e++) cw[e] = G[(long)e * ntheta + t] * w[o] / k2;
+    if (mirror) {
+        const int odd[8] = {2, 3, 6, 7, 8, 9, 12, 13};
+        for (int b = 0; b < 8; b++) cw[odd[b]] = 0.0;
+    }
+    // coefficients of |S_i|^2, Re(S_i conj S_j), Im(S_i conj S_j) in sum_e cw_e M_e
+    const double A1 = 0.5 * (cw[0] - cw[1] - cw[4] + cw[5]), A2 = 0.5 * (cw[0] + cw[1] + cw[4] + cw[5]);
+    const double A3 = 0.5 * (cw[0] - cw[1] + cw[4] - cw[5]), A4 = 0.5 * (cw[0] + cw[1] - cw[4] - cw[5]);
+    double sig[5][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    auto quad_abs = [&](int i, double cf) {  // cf |S_i|^2
+        sig[i][0] += cf * S[i][0];
+        sig[i][1] -= cf * S[i][1];
+    };
+    auto quad_re = [&](int i, int j, double cf) {  // cf Re(S_i conj S_j)
+        sig[i][0] += 0.5 * cf * S[j][0];
+        sig[i][1] -= 0.5 * cf * S[j][1];
+        sig[j][0] += 0.5 * cf * S[i][0];
+        sig[j][1] -= 0.5 * cf * S[i][1];
+    };
+    auto quad_im = [&](int i, int j, double cf) {  // cf Im(S_i conj S_j): d/dS_i = conj S_j/(2i), d/dS_j = -conj S_i/(2i)
+        // conj(S)/(2i) = (-i/2)(Sr - i Si) = (-Si/2, -Sr/2)
+        sig[i][0] += cf * (-0.5 * S[j][1]);
+        sig[i][1] += cf * (-0.5 * S[j][0]);
+        sig[j][0] -= cf * (-0.5 * S[i][1]);
+        sig[j][1] -= cf * (-0.5 * S[i][0]);
+    };
+    quad_abs(1, A1);
+    quad_abs(2, A2);
+    quad_abs(3, A3);
+    quad_abs(4, A4);
+    quad_re(2, 3, cw[2] + cw[6]);
+    quad_re(1, 4, cw[2] - cw[6]);
+    quad_im(2, 3, cw[3] + cw[7]);
+    quad_im(1, 4, cw[7] - cw[3]);
+    quad_re(2, 4, cw[8] + cw[9]);
+    quad_re(1, 3, cw[8] - cw[9]);
+    quad_re(1, 2, cw[10] + cw[15]);
+    quad_re(3, 4, cw[10] - cw[15]);
+    quad_im(2, 1, cw[11]);
+    quad_im(4, 3, cw[11]);
+    quad_im(4, 2, cw[12] + cw[13]);
+    quad_im(1, 3, cw[12] - cw[13]);
+    quad_im(1, 2, cw[14]);
+    quad_im(3, 4, -cw[14]);
+    // explicit dL/dk: 2 Re sum sigma (2/k) S  and  -(2/k) sum cw M
+    double sS[2] = {0.0, 0.0}, cM = 0.0;
+    for (int l = 1; l <= 4; l++) {
+        sS[0] += sig[l][0] * S[l][0] - sig[l][1] * S[l][1];
+        sS[1] += sig[l][0] * S[l][1] + sig[l][1] * S[l][0];
+    }
+    {
+        const double a1 = S[1][0] * S[1][0] + S[1][1] * S[1][1], a2 = S[2][0] * S[2][0] + S[2][1] * S[2][1];
+        const double a3 = S[3][0] * S[3][0] + S[3][1] * S[3][1], a4 = S[4][0] * S[4][0] + S[4][1] * S[4][1];
+        cM = A1 * a1 + A2 * a2 + A3 * a3 + A4 * a4;
+        auto re_ij = [&](int i, int j) { return S[i][0] * S[j][0] + S[i][1] * S[j][1]; };
+        auto im_ij = [&](int i, int j) { return S[i][1] * S[j][0] - S[i][0] * S[j][1]; };
+        cM += (cw[2] + cw[6]) * re_ij(2, 3) + (cw[2] - cw[6]) * re_ij(1, 4) + (cw[3] + cw[7]) * im_ij(2, 3) +
+              (cw[7] - cw[3]) * im_ij(1, 4) + (cw[8] + cw[9]) * re_ij(2, 4) + (cw[8] - cw[9]) * re_ij(1, 3) +
+              (cw[10] + cw[15]) * re_ij(1, 2) + (cw[10] - cw[15]) * re_ij(3, 4) + cw[11] * (im_ij(2, 1) + im_ij(4, 3)) +
+              (cw[12] + cw[13]) * im_ij(4, 2) + (cw[12] - cw[13]) * im_ij(1, 3) + cw[14] * (im_ij(1, 2) - im_ij(3, 4));
+    }
+    // 2/k (complex)
+    const double tk[2] = {2.0 * kr / k2, -2.0 * ki / k2};
+    expl[idx] = 2.0 * (tk[0] * sS[0] - tk[1] * sS[1]) - tk[0] * cM;
+    // dL/dF of both calls, then dL/dV = (-ik/(4 pi)) dL/dF
+    const double* r = g;
+    const double* ep = g + 3;
+    const double* epp = g + 6;
+    const int lp[2][2] = {{2, 4}, {3, 1}};  // call 0: S2 (e_par), S4 (e_perp); call 1: S3, S1
+    const double pv[2] = {mik[0] / (4.0 * kPi), mik[1] / (4.0 * kPi)};
+    for (int call = 0; call < 2; call++) {
+        double dF[3][2];
+        for (int e = 0; e < 3; e++) {
+            const double* s1 = sig[lp[call][0]];
+            const double* s2 = sig[lp[call][1]];
+            const double vr = s1[0] * ep[e] + s2[0] * epp[e], vi = s1[1] * ep[e] + s2[1] * epp[e];
+            dF[e][0] = mik[0] * vr - mik[1] * vi;
+            dF[e][1] = mik[0] * vi + mik[1] * vr;
+        }
+        double dV[3][2];
+        for (int e = 0; e < 3; e++) {
+            dV[e][0] = pv[0] * dF[e][0] - pv[1] * dF[e][1];
+            dV[e][1] = pv[0] * dF[e][1] + pv[1] * dF[e][0];
+        }
+        const double rvr = r[0] * dV[0][0] + r[1] * dV[1][0] + r[2] * dV[2][0];
+        const double rvi = r[0] * dV[0][1] + r[1] * dV[1][1] + r[2] * dV[2][1];
+        double* out = coef + (long)idx * 24 + call * 12;
+        for (int e = 0; e < 3; e++) {
+            const int e1 = (e + 1) % 3, e2 = (e + 2) % 3;
+            out[2 * e] = eta * (dV[e][0] - r[e] * rvr);
+            out[2 * e + 1] = eta * (dV[e][1] - r[e] * rvi);
+            out[6 + 2 * e] = r[e1] * dV[e2][0] - r[e2] * dV[e1][0];  // rhat x dL/dV
+            out[6 + 2 * e + 1] = r[e1] * dV[e2][1] - r[e2] * dV[e1][1];
+        }
+    }
+}
+
+// Transpose of the far-field sums.  Block (o, chunk of 256 points), thread =
+// point x: A_c(x) = sum_d coef_{o,d,c} e^{-ik rhat_d.x} (dL/d current at x of
+// call c = 2o, 2o+1) into acur, and the explicit phase term
+// 2 Re sum_c cur_c(x) . sum_d coef_{o,d,c} (-i rhat_d.x) e^{-ik rhat_d.x}
+// summed over the block into epart[o * gridDim.y + chunk].
+constexpr int kDirTile = 64;
+__global__ __launch_bounds__(256) void k_ff_adj(long P, int n_calls, int ndir, const double4* __restrict__ qp,
+                                                const double* __restrict__ cur, const double* __restrict__ coef,
+                                                const double* __restrict__ rh, double kr, double ki,
+                                                double* __restrict__ acur, double* __restrict__ epart) {
+    __shared__ double sc[kDirTile][24];
+    __shared__ double sr[kDirTile][3];
+    __shared__ double red[256];
+    const int o = blockIdx.x;
+    const long x = (long)blockIdx.y * 256 + threadIdx.x;
+    const bool ok = x < P;
+    double4 X = ok ? qp[x] : make_double4(0, 0, 0, 0);
+    double A[24], Ad[24];
+    for (int e = 0; e < 24; e++) A[e] = Ad[e] = 0.0;
+    for (int d0 = 0; d0 < ndir; d0 += kDirTile) {
+        const int nd = min(kDirTile, ndir - d0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nd * 24; e += 256)
+            sc[e / 24][e % 24] = coef[((long)o * ndir + d0 + e / 24) * 24 + e % 24];
+        for (int e = threadIdx.x; e < nd * 3; e += 256) sr[e / 3][e % 3] = rh[((long)o * ndir + d0 + e / 3) * 3 + e % 3];
+        __syncthreads();
+        if (ok) {
+            for (int d = 0; d < nd; d++) {
+                const double rd = sr[d][0] * X.x + sr[d][1] * X.y + sr[d][2] * X.z;
+                double sn, cs;
+                sincos(kr * rd, &sn, &cs);
+                double pr = cs, pi = -sn;  // e^{-ik rd}
+                if (ki != 0.0) {
+                    const double ea = exp(ki * rd);
+                    pr *= ea;
+                    pi *= ea;
+                }
+                const double qr = rd * pi, qi = -rd * pr;  // (-i rd) e^{-ik rd}
+#pragma unroll
+                for (int k = 0; k < 12; k++) {
+                    const double ar = sc[d][2 * k], ai = sc[d][2 * k + 1];
+                    A[2 * k] = fma(pr, ar, fma(-pi, ai, A[2 * k]));
+                    A[2 * k + 1] = fma(pr, ai, fma(pi, ar, A[2 * k + 1]));
+                    Ad[2 * k] = fma(qr, ar, fma(-qi, ai, Ad[2 * k]));
+                    Ad[2 * k + 1] = fma(qr, ai, fma(qi, ar, Ad[2 * k + 1]));
+                }
+            }
+        }
+    }
+    double ex = 0.0;
+    if (ok) {
+        for (int call = 0; call < 2; call++) {
+            const int c = 2 * o + call;
+            if (c >= n_calls) break;
+            const double* cu = cur + ((long)c * P + x) * 12;
+            double* out = acur + ((long)c * P + x) * 12;
+            for (int k = 0; k < 6; k++) {
+                const double ar = A[call * 12 + 2 * k], ai = A[call * 12 + 2 * k + 1];
+                out[2 * k] = ar;
+                out[2 * k + 1] = ai;
+                ex += cu[2 * k] * Ad[call * 12 + 2 * k] - cu[2 * k + 1] * Ad[call * 12 + 2 * k + 1];
+            }
+        }
+    }
+    red[threadIdx.x] = 2.0 * ex;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) epart[(long)o * gridDim.y + blockIdx.y] = red[0];
+}
+
+// Transpose of k_current: g[n][c] = sum over the halves (t, a) of RWG n and
+// the points x of t of f_a(x) jw_x . A_{c,J}(x); g[N + n][c] likewise with A_M.
+__global__ __launch_bounds__(256) void k_current_adj(int N, int Nq, int n_calls, int ldr,
+                                                     const double4* __restrict__ qp, const double* __restrict__ hc,
+                                                     const double* __restrict__ hp, const int4* __restrict__ rwg_half,
+                                                     const double* __restrict__ acur, long P,
+                                                     double* __restrict__ gr, double* __restrict__ gi) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)N * n_calls) return;
+    const int n = (int)(idx / n_calls), c = (int)(idx % n_calls);
+    const int4 h = rwg_half[n];
+    double J[2] = {0, 0}, M[2] = {0, 0};
+    for (int half = 0; half < 2; half++) {
+        const int t = half == 0 ? h.x : h.z, a = half == 0 ? h.y : h.w;
+        const double ca = hc[t * 3 + a];
+        const double* pa = hp + (t * 3 + a) * 3;
+        for (int q = 0; q < Nq; q++) {
+            const long x = (long)t * Nq + q;
+            const double4 r = qp[x];
+            const double f[3] = {(r.x - pa[0]) * ca * r.w, (r.y - pa[1]) * ca * r.w, (r.z - pa[2]) * ca * r.w};
+            const double* A = acur + ((long)c * P + x) * 12;
+            for (int e = 0; e < 3; e++) {
+                J[0] += f[e] * A[2 * e];
+                J[1] += f[e] * A[2 * e + 1];
+                M[0] += f[e] * A[6 + 2 * e];
+                M[1] += f[e] * A[6 + 2 * e + 1];
+            }
+        }
+    }
+    gr[(long)n * ldr + c] = J[0];
+    gi[(long)n * ldr + c] = J[1];
+    gr[(long)(N + n) * ldr + c] = M[0];
+    gi[(long)(N + n) * ldr + c] = M[1];
+}
+
 size_t farfield_scratch_bytes(const DevMesh& d, int n_calls, int n_orient, int ndir) {
     const long P = (long)d.T * d.Nq;
     const int nchunk = (int)((P + kChunk - 1) / kChunk);
@@ -344,11 +595,32 @@ int rhs_launch(bem_ctx* c, double kr, double ki, double eta, const double* khat,
     const long cnt = (long)c->npad * c->ldr;
     const int kt = kt_mark(c, KS_RHS);
     hipLaunchKernelGGL(k_rhs, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, d.N, d.Nq, c->npad,
-                       c->ldr, nrhs, d.qp, d.hc, d.hp, d.rwg_half, static_cast<const double*>(s), kr, ki, c->Br,
+                       c->ldr, nrhs, d.qp, d.hc, d.hp, d.rwg_half, static_cast<const double*>(s), kr, ki, 0, c->Br,
                        c->Bi);
     kt_close(c, kt, (double)d.N * nrhs * 2 * d.Nq, (double)cnt * 16.0);
     BEM_HIP(c, hipGetLastError());
     BEM_HIP(c, hipStreamSynchronize(c->stream));
+    c->rhs_prm.swap(prm);
+    c->rhs_k[0] = kr;
+    c->rhs_k[1] = ki;
+    c->rhs_eta = eta;
+    return 0;
+}
+
+int rhs_dk_launch(bem_ctx* c, double* dBr, double* dBi) {
+    const int nrhs = (int)(c->rhs_prm.size() / 9);
+    if (nrhs < 1 || nrhs > c->ldr) return bem_fail(c, BEM_ERR_STATE, "rhs derivative: no plane waves stored");
+    double* s = nullptr;
+    if (hipMalloc(&s, c->rhs_prm.size() * sizeof(double)) != hipSuccess)
+        return bem_fail(c, BEM_ERR_NOMEM, "rhs derivative: parameters");
+    BEM_HIP(c, hipMemcpy(s, c->rhs_prm.data(), c->rhs_prm.size() * sizeof(double), hipMemcpyHostToDevice));
+    const DevMesh& d = c->dm;
+    const long cnt = (long)c->npad * c->ldr;
+    hipLaunchKernelGGL(k_rhs, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, d.N, d.Nq, c->npad,
+                       c->ldr, nrhs, d.qp, d.hc, d.hp, d.rwg_half, s, c->rhs_k[0], c->rhs_k[1], 1, dBr, dBi);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(s);
+    if (e != hipSuccess) return bem_fail(c, BEM_ERR_HIP, std::string("rhs derivative: ") + hipGetErrorString(e));
     return 0;
 }
 
@@ -367,10 +639,12 @@ int ff_launch(bem_ctx* c, const double* dJr, const double* dJi, const double* dM
     return 0;
 }
 
-int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* RT, const double* w, int n_orient,
-                   const double* theta, int ntheta, int mirror, double* M_avg, double* M_orient) {
-    // directions and polarisation vectors in the body frame (main.cpp:766-799)
-    std::vector<double> rh((size_t)n_orient * ntheta * 3), geo((size_t)n_orient * ntheta * 9);
+// Directions rhat (n_orient x ntheta x 3) and (rhat, e_par, e_perp) per
+// (orientation, angle) in the body frame (main.cpp:766-799).
+static void build_dirs(const double* RT, int n_orient, const double* theta, int ntheta, std::vector<double>& rh,
+                       std::vector<double>& geo) {
+    rh.assign((size_t)n_orient * ntheta * 3, 0.0);
+    geo.assign((size_t)n_orient * ntheta * 9, 0.0);
     for (int o = 0; o < n_orient; o++) {
         const double* R = RT + 9 * o;
         auto mul = [R](const double v[3], double* out) {
@@ -391,6 +665,12 @@ int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* R
             rh[((size_t)o * ntheta + t) * 3 + 2] = g[2];
         }
     }
+}
+
+int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* RT, const double* w, int n_orient,
+                   const double* theta, int ntheta, int mirror, double* M_avg, double* M_orient) {
+    std::vector<double> rh, geo;
+    build_dirs(RT, n_orient, theta, ntheta, rh, geo);
     const int n_calls = 2 * n_orient;
     const size_t fvb = (size_t)n_calls * ntheta * 6 * sizeof(double);
     const size_t mob = (size_t)n_orient * 16 * ntheta * sizeof(double);
@@ -422,5 +702,70 @@ int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* R
     BEM_HIP(c, hipMemcpyAsync(M_avg, dM, 16 * (size_t)ntheta * 8, hipMemcpyDeviceToHost, s));
     if (M_orient) BEM_HIP(c, hipMemcpyAsync(M_orient, dMo, mob, hipMemcpyDeviceToHost, s));
     BEM_HIP(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+int ff_adjoint_launch(bem_ctx* c, const double* dL_dM, double* gr, double* gi, double* explicit_dk) {
+    const int n_orient = c->last_norient, ntheta = c->last_ntheta, n_calls = 2 * n_orient;
+    const double kr = c->last_kext[0], ki = c->last_kext[1], eta = c->last_eta_ext;
+    std::vector<double> rh, geo;
+    build_dirs(c->last_RT.data(), n_orient, c->last_theta.data(), ntheta, rh, geo);
+    const DevMesh& d = c->dm;
+    const long P = (long)d.T * d.Nq;
+    const int nchunk = (int)((P + 255) / 256);
+    const size_t nfv = (size_t)n_calls * ntheta * 6, ncoef = (size_t)n_orient * ntheta * 24;
+    const size_t nacur = (size_t)n_calls * P * 12, nsmall = rh.size() + geo.size() + (size_t)n_orient +
+                                                           16 * (size_t)ntheta + (size_t)n_orient * ntheta +
+                                                           (size_t)n_orient * nchunk;
+    const size_t bytes = farfield_scratch_bytes(d, n_calls, n_orient, ntheta) +
+                         (nfv + ncoef + nacur + nsmall + 64) * sizeof(double);
+    char* base = nullptr;
+    if (hipMalloc(&base, bytes) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "far-field adjoint: work area");
+    double* work = reinterpret_cast<double*>(base);  // cur + partials of farfield_core first
+    double* dFv = work + farfield_scratch_bytes(d, n_calls, n_orient, ntheta) / sizeof(double);
+    double* dcoef = dFv + nfv;
+    double* dacur = dcoef + ncoef;
+    double* drh = dacur + nacur;
+    double* dgeo = drh + rh.size();
+    double* dw = dgeo + geo.size();
+    double* dG = dw + n_orient;
+    double* dexpl = dG + 16 * (size_t)ntheta;
+    double* depart = dexpl + (size_t)n_orient * ntheta;
+    hipStream_t s = c->stream;
+    int rc = 0;
+    auto fail = [&](hipError_t e, const char* what) {
+        rc = bem_fail(c, BEM_ERR_HIP, std::string("far-field adjoint: ") + what + ": " + hipGetErrorString(e));
+    };
+    hipError_t e = hipMemcpyAsync(drh, rh.data(), rh.size() * 8, hipMemcpyHostToDevice, s);
+    if (!e) e = hipMemcpyAsync(dgeo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice, s);
+    if (!e) e = hipMemcpyAsync(dw, c->last_w.data(), (size_t)n_orient * 8, hipMemcpyHostToDevice, s);
+    if (!e) e = hipMemcpyAsync(dG, dL_dM, 16 * (size_t)ntheta * 8, hipMemcpyHostToDevice, s);
+    if (!e) e = hipMemsetAsync(gr, 0, (size_t)c->npad * c->ldr * 8, s);
+    if (!e) e = hipMemsetAsync(gi, 0, (size_t)c->npad * c->ldr * 8, s);
+    if (e) fail(e, "upload");
+    std::vector<double> hexpl((size_t)n_orient * ntheta), hep((size_t)n_orient * nchunk);
+    if (!rc) {
+        const int N = c->rwg.N;
+        const long ld = c->ldr;
+        // forward far field again (currents in `work`, field in dFv)
+        farfield_core(c, c->Xr, c->Xi, c->Xr + (long)N * ld, c->Xi + (long)N * ld, 1, ld, drh, kr, ki, eta, n_calls,
+                      n_orient, ntheta, dFv, reinterpret_cast<char*>(work));
+        hipLaunchKernelGGL(k_mueller_bwd, dim3((n_orient * ntheta + 255) / 256), dim3(256), 0, s, n_orient, ntheta,
+                           dFv, dgeo, dG, dw, kr, ki, eta, c->last_mirror, dcoef, dexpl);
+        hipLaunchKernelGGL(k_ff_adj, dim3(n_orient, nchunk), dim3(256), 0, s, P, n_calls, ntheta, d.qp, work, dcoef,
+                           drh, kr, ki, dacur, depart);
+        hipLaunchKernelGGL(k_current_adj, dim3((unsigned)(((long)N * n_calls + 255) / 256)), dim3(256), 0, s, N,
+                           d.Nq, n_calls, c->ldr, d.qp, d.hc, d.hp, d.rwg_half, dacur, P, gr, gi);
+        e = hipMemcpyAsync(hexpl.data(), dexpl, hexpl.size() * 8, hipMemcpyDeviceToHost, s);
+        if (!e) e = hipMemcpyAsync(hep.data(), depart, hep.size() * 8, hipMemcpyDeviceToHost, s);
+        if (!e) e = hipStreamSynchronize(s);
+        if (e) fail(e, "kernels");
+    }
+    (void)hipFree(base);
+    if (rc) return rc;
+    double sum = 0.0;
+    for (double v : hexpl) sum += v;
+    for (double v : hep) sum += v;
+    *explicit_dk = sum;
     return 0;
 }

@@ -74,23 +74,36 @@ __device__ __forceinline__ void store_row(T* __restrict__ p, const T (&x)[PB]) {
     }
 }
 
+// Wave-wide maximum of an unsigned value (DPP reduction of the device library).
+extern "C" __device__ unsigned __ockl_wfred_max_u32(unsigned);
+
+// Order-preserving 32-bit key of a non-negative magnitude; 0 is reserved for
+// "no candidate".  Doubles keep their sign, exponent and 20 leading mantissa
+// bits: rows closer than 2^-20 in magnitude tie and the lower one wins.
+__device__ __forceinline__ unsigned mag_key(float m) { return __float_as_uint(m) + 1u; }
+__device__ __forceinline__ unsigned mag_key(double m) {
+    return (unsigned)((unsigned long long)__double_as_longlong(m) >> 32) + 1u;
+}
+
 // Exchange area of the register-resident elimination (double-buffered by
 // step parity so each step needs one barrier).
 template <typename T>
 struct ElimLds {
     T pr[2][4][PB], pi[2][4][PB];  // best row of each wave
-    T bv[2][4];
-    int bp[2][4];
+    unsigned bk[2][4];             // its key
+    int bp[2][4];                  // its thread
     int oid[PB], osrc[PB];  // chosen rows in pivot order
     int last;
     int zmin;  // first zero pivot of the root (1-based)
 };
 
 // Partial pivoting over the block's rows, one row per thread in registers
-// (rows t >= nrows take no part), with implicit row interchanges: rows stay
-// with their thread and a chosen row is only marked.  Step jc: the largest
-// |x_jc|^2 among the rows not chosen yet (ties: lower thread) becomes pivot
-// jc, and the remaining rows are eliminated (multiplier stored in x_jc).
+// (rows t >= nrows take no part; nrows >= PB), with implicit row
+// interchanges: rows stay with their thread and a chosen row is only marked.
+// Step jc: the largest |x_jc|^2 among the rows not chosen yet (by mag_key;
+// ties: lower thread) becomes pivot jc, and the remaining rows are eliminated
+// (multiplier stored in x_jc).  An all-zero column chooses the lowest open
+// row and eliminates nothing.
 // Afterwards the thread of pivot r holds row r of L11 \ U11 and returns r
 // (-1 for rows not chosen); oid/osrc list the chosen rows in pivot order.
 template <typename T>
@@ -102,17 +115,9 @@ __device__ __forceinline__ int elim_rows(ElimLds<T>& L, T (&xr)[PB], T (&xi)[PB]
     for (int jc = 0; jc < PB; jc++) {
         const int b = jc & 1;
         const bool open = valid && mine < 0;
-        T v = open ? mag2(xr[jc], xi[jc]) : T(-1);
-        int p = t;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(v, off);
-            const int op = __shfl_xor(p, off);
-            if (ov > v || (ov == v && op < p)) {
-                v = ov;
-                p = op;
-            }
-        }
+        const unsigned key = open ? mag_key(mag2(xr[jc], xi[jc])) : 0u;
+        const unsigned wk = __ockl_wfred_max_u32(key);
+        const int p = wave * 64 + __ffsll((long long)__ballot(key == wk)) - 1;  // ties: lowest thread
         if (t == p) {
 #pragma unroll
             for (int c = 0; c < PB; c++)
@@ -120,25 +125,24 @@ __device__ __forceinline__ int elim_rows(ElimLds<T>& L, T (&xr)[PB], T (&xi)[PB]
                     L.pr[b][wave][c] = xr[c];
                     L.pi[b][wave][c] = xi[c];
                 }
-            L.bv[b][wave] = v;
+            L.bk[b][wave] = wk;
             L.bp[b][wave] = p;
         }
         __syncthreads();
-        T best = L.bv[b][0];
+        unsigned best = L.bk[b][0];
         int pp = L.bp[b][0], pw = 0;
 #pragma unroll
         for (int w = 1; w < 4; w++) {
-            const T bw = L.bv[b][w];
-            const int qw = L.bp[b][w];
-            if (bw > best || (bw == best && qw < pp)) {
+            const unsigned bw = L.bk[b][w];
+            if (bw > best) {
                 best = bw;
-                pp = qw;
+                pp = L.bp[b][w];
                 pw = w;
             }
         }
         if (t == pp) {
             mine = jc;
-        } else if (open && best > T(0)) {
+        } else if (open && best > 1u) {
             T lr, li;
             cdiv(xr[jc], xi[jc], L.pr[b][pw][jc], L.pi[b][pw][jc], lr, li);
             xr[jc] = lr;
@@ -171,7 +175,7 @@ __device__ void root_swaps(const int* __restrict__ oid, int j0, int* __restrict_
     int rows = -1, cont = -1, nt = 0;
     for (int c = 0; c < PB; c++) {
         const int x = j0 + c;
-        const int y = __shfl(pos, c);
+        const int y = __builtin_amdgcn_readlane(pos, c);
         if (lane == 0) ipiv[x] = y;
         if (lane > c && pos == x) pos = y;  // the row at x moves to y
         if (y == x) continue;
@@ -192,7 +196,7 @@ __device__ void root_swaps(const int* __restrict__ oid, int j0, int* __restrict_
         } else {
             iy = __ffsll((long long)m) - 1;
         }
-        const int cx = __shfl(cont, ix), cy = __shfl(cont, iy);
+        const int cx = __builtin_amdgcn_readlane(cont, ix), cy = __builtin_amdgcn_readlane(cont, iy);
         if (lane == ix) cont = cy;
         if (lane == iy) cont = cx;
     }

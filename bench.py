@@ -48,6 +48,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, kernel_class):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of
+    known size), or None when that workload has not been profiled."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_class}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def mueller_err(M, Mref):
     return float((np.abs(M - Mref) / np.abs(Mref[0, 0])[None, None, :]).max())
 
@@ -96,6 +109,9 @@ def main():
     ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the steps, copy MIB MiB device-to-device with one torch kernel (a known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters of profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by orientation, "
@@ -114,6 +130,16 @@ def main():
     k_ext, k_int, eta_int = complex(ka), ka * m, 1.0 / abs(m)
 
     ctx = bem_amd.Context(0)
+    if args.copy_calibration > 0:
+        import torch
+        nel = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
+        del src, dst
     N = ctx.set_mesh(verts, tris, 7)
     n = 2 * N
     log(f"workload {args.workload}: {shape} ref {ref} T={len(tris)} N={N} 2N={n} ka={ka} m={m} "
@@ -174,7 +200,7 @@ def main():
         ach = kflops / (kms * 1e-3) / 1e12
         roof = {"kernel": "k_cgemm<float> (3M complex GEMM, LU trailing updates + solves)" if dom == "gemm_fp32_3m"
                 else "k_cgemm<double>", "bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[dom],
-                "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom], "traffic": None,
+                "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[dom], "traffic": measured_traffic(args.workload, dom),
                 "launches": int(launches), "avg_launch_ms": kms / max(launches, 1)}
     else:
         ach = kbytes / (kms * 1e-3) / 1e9
@@ -195,7 +221,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 LU/solve on MFMA (3M) + fp64 refinement, assembly sums, far field",
+        "dtype": "fp32",
+        "precision": "fp32 LU/solve on MFMA (3M) + fp64 refinement; fp64 assembly sums, far field, Mueller",
         "data": "synthetic (deterministic mesh and plane waves)",
         "config": {
             "workload": (f"{shape} {'AR ' + str(param) if shape == 'hex' else 'r=1'} ref {ref}: T={len(tris)}, "

@@ -34,3 +34,9 @@ def oracle():
     from oracle import oracle as o
     o.lib()
     return o
+
+
+@pytest.fixture(scope="session")
+def bem():
+    import bem_amd
+    return bem_amd

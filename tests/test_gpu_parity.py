@@ -10,12 +10,6 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def bem():
-    import bem_amd
-    return bem_amd
-
-
-@pytest.fixture(scope="module")
 def ctx(bem):
     c = bem.Context(0)
     yield c
