@@ -70,6 +70,7 @@ def lib():
     sig = {
         "bem_create": ([i], vp), "bem_destroy": ([vp], None), "bem_last_error": ([vp], ctypes.c_char_p),
         "bem_timings": ([vp, _dp, i], i), "bem_kernel_stats": ([vp, i, _dp, i], i),
+        "bem_autotune": ([vp, i, _dp, i], i),
         "bem_mesh_counts": ([i, i, _ip, _ip], i), "bem_make_mesh": ([i, i, d, _dp, _ip], i),
         "bem_orientations": ([i, i, i, i, i, i, i, _dp, _dp], i),
         "bem_set_mesh": ([vp, _dp, i, _ip, i, i], i), "bem_num_rwg": ([vp], i),
@@ -271,6 +272,21 @@ class Context:
         d = _cd(dL_dM)
         self._check(self._L.bem_mueller_grad(self._h, _p(d), _p(g)), "mueller_grad")
         return g
+
+    GEMM_TILES = ("128x128", "128x64", "64x64")  # FP32 configurations
+    GEMM_TILES64 = ("128x64", "64x64", "64x32")  # FP64 configurations
+
+    def autotune(self, nrhs=128):
+        """Times the GEMM tile configurations at this system's shapes and keeps
+        the fastest; returns the measured TFLOP/s and the choices."""
+        out = np.zeros(16)
+        n = self._L.bem_autotune(self._h, int(nrhs), _p(out), 16)
+        self._check(n, "autotune")
+        r = out[:12].reshape(4, 3)
+        return {"fp32_trailing": dict(zip(self.GEMM_TILES, r[0])), "fp32_solve": dict(zip(self.GEMM_TILES, r[1])),
+                "fp64_trailing": dict(zip(self.GEMM_TILES64, r[2])), "fp64_solve": dict(zip(self.GEMM_TILES64, r[3])),
+                "fp32_choice": (self.GEMM_TILES[int(out[12])], self.GEMM_TILES[int(out[13])]),
+                "fp64_choice": (self.GEMM_TILES64[int(out[14])], self.GEMM_TILES64[int(out[15])])}
 
     KERNEL_CLASSES = ("gemm_fp32_3m", "gemm_fp64", "asm_far", "asm_near", "asm_gather", "tslu", "laswp", "panel",
                       "trsm", "farfield", "rhs", "other")

@@ -252,6 +252,14 @@ int bem_kernel_stats(bem_ctx* c, int mode, double* out, int n) {
     return m;
 }
 
+int bem_autotune(bem_ctx* c, int nrhs, double* out, int nout) {
+    if (!c || nrhs < 1 || nout < 0) return bem_fail(c, BEM_ERR_ARG, "autotune: bad arguments");
+    if (c->npad <= 0) return bem_fail(c, BEM_ERR_STATE, "autotune: no system size yet (set a mesh or a matrix)");
+    (void)hipSetDevice(c->device);
+    double local[32];
+    return gemm_autotune_launch(c, c->npad, nrhs, out ? out : local, out ? nout : 32);
+}
+
 int bem_mesh_counts(int shape, int refinements, int* nv, int* nt) {
     if (!nv || !nt || refinements < 0 || refinements > 8 || (shape != 0 && shape != 1)) return BEM_ERR_ARG;
     long t = shape == 1 ? 24 : 20;

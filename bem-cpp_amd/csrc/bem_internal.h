@@ -78,8 +78,20 @@ struct KernelTimer {
     long launches[KS_COUNT] = {};
 };
 
+// GEMM tile choice of one precision (configurations of kernels/gemm.h):
+// `large` for products with at least `fill` blocks in it, else `skinny`.
+struct GemmPlan {
+    int large = 0, skinny = 2;
+    long fill = 512;
+    bool tuned = false;
+    int npad = 0;  // system size the plan was tuned for
+    GemmPlan() = default;
+    GemmPlan(int l, int s) : large(l), skinny(s) {}
+};
+
 struct bem_ctx {
     int device = 0;
+    GemmPlan gemm_plan[2] = {GemmPlan(0, 2), GemmPlan(1, 2)};  // [0] FP32 3M, [1] FP64 (defaults before tuning)
     KernelTimer kt;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // panel stream of the look-ahead factorisation
@@ -166,6 +178,11 @@ int ff_launch(bem_ctx* c, const double* dJr, const double* dJi, const double* dM
 int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* RT, const double* w, int n_orient,
                    const double* theta, int ntheta, int mirror, double* M_avg, double* M_orient);
 int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad);
+// Times the GEMM tile configurations at the shapes of an npad system with
+// nrhs right-hand sides and sets c->gemm_plan; out (nullable) receives the
+// measured TFLOP/s per (precision, shape, configuration).
+int gemm_autotune_launch(bem_ctx* c, int npad, int nrhs, double* out, int nout);
+int gemm_ensure_tuned(bem_ctx* c, bool f32);
 // d B / d k_ext of the stored plane waves into (dBr, dBi) (npad x ldr).
 int rhs_dk_launch(bem_ctx* c, double* dBr, double* dBi);
 // Far-field part of the adjoint for the last bem_orient_mueller call: given

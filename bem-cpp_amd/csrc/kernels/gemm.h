@@ -31,25 +31,60 @@ __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-template <typename T>
+// Tile configurations, indexed per precision (0, 1, 2): block BM x BN, wave
+// tile WM x WN (4 waves in 2 x 2), K slice BK.  MINB: workgroups per CU the
+// register allocation must allow (two per CU let one block's prologue and C
+// update overlap another's MFMA stream).  Which configuration a launch uses
+// is decided by the context's plan (gemm_plan in bem_ctx), set by the
+// autotuner from measured times at the context's shapes.
+template <typename T, int CFG>
 struct GemmCfg;
 template <>
-struct GemmCfg<float> {
-    static constexpr int BM = 128, BN = 128, BK = 16, WM = 64, WN = 64, PAD = 16;
+struct GemmCfg<float, 0> {
+    static constexpr int BM = 128, BN = 128, BK = 16, WM = 64, WN = 64, PAD = 16, MINB = 1;
     typedef v4f acc_t;
     typedef float4 vec_t;  // 16-byte global loads
     static constexpr int VEC = 4;
 };
 template <>
-struct GemmCfg<double> {
-    static constexpr int BM = 128, BN = 64, BK = 16, WM = 64, WN = 32, PAD = 8;
+struct GemmCfg<float, 1> {
+    static constexpr int BM = 128, BN = 64, BK = 16, WM = 64, WN = 32, PAD = 16, MINB = 2;
+    typedef v4f acc_t;
+    typedef float4 vec_t;
+    static constexpr int VEC = 4;
+};
+template <>
+struct GemmCfg<float, 2> {
+    static constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32, PAD = 16, MINB = 2;
+    typedef v4f acc_t;
+    typedef float4 vec_t;
+    static constexpr int VEC = 4;
+};
+template <>
+struct GemmCfg<double, 0> {
+    static constexpr int BM = 128, BN = 64, BK = 16, WM = 64, WN = 32, PAD = 8, MINB = 1;
     typedef v4d acc_t;
     typedef double2 vec_t;
     static constexpr int VEC = 2;
 };
+template <>
+struct GemmCfg<double, 1> {
+    static constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32, PAD = 8, MINB = 2;
+    typedef v4d acc_t;
+    typedef double2 vec_t;
+    static constexpr int VEC = 2;
+};
+template <>
+struct GemmCfg<double, 2> {
+    static constexpr int BM = 64, BN = 32, BK = 16, WM = 32, WN = 16, PAD = 8, MINB = 2;
+    typedef v4d acc_t;
+    typedef double2 vec_t;
+    static constexpr int VEC = 2;
+};
+constexpr int kGemmCfgs = 3;
 
 template <typename T>
-__device__ __forceinline__ void vec_split(const typename GemmCfg<T>::vec_t& v, T* out);
+__device__ __forceinline__ void vec_split(const typename GemmCfg<T, 0>::vec_t& v, T* out);
 template <>
 __device__ __forceinline__ void vec_split<float>(const float4& v, float* o) {
     o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
@@ -59,12 +94,12 @@ __device__ __forceinline__ void vec_split<double>(const double2& v, double* o) {
     o[0] = v.x; o[1] = v.y;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_cgemm(int M, int N, int K, const T* __restrict__ Ar,
+template <typename T, int CFG>
+__global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, int N, int K, const T* __restrict__ Ar,
                                                const T* __restrict__ Ai, long lda, const T* __restrict__ Br,
                                                const T* __restrict__ Bi, long ldb, T* __restrict__ Cr,
                                                T* __restrict__ Ci, long ldc, int mode) {
-    typedef GemmCfg<T> C;
+    typedef GemmCfg<T, CFG> C;
     constexpr int BM = C::BM, BN = C::BN, BK = C::BK, WM = C::WM, WN = C::WN, VEC = C::VEC;
     constexpr int TM = WM / 16, TN = WN / 16;
     constexpr int SA = BM + C::PAD, SB = BN + C::PAD;
@@ -215,19 +250,51 @@ __global__ __launch_bounds__(256) void k_cgemm(int M, int N, int K, const T* __r
         }
 }
 
-// Host launcher.  K must be a multiple of BK (16); N a multiple of the vector
-// width (4 floats / 2 doubles) and every leading dimension a multiple of it.
-template <typename T>
-inline void cgemm(hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br, const T* Bi,
-                  long ldb, T* Cr, T* Ci, long ldc, int mode) {
-    if (M <= 0 || N <= 0 || K <= 0) return;
-    typedef GemmCfg<T> C;
+// Host launcher of configuration cfg.  K must be a multiple of BK (16); N a
+// multiple of the vector width (4 floats / 2 doubles) and every leading
+// dimension a multiple of it.
+template <typename T, int CFG>
+inline void cgemm_cfg(hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
+                      const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
+    typedef GemmCfg<T, CFG> C;
     dim3 grid((N + C::BN - 1) / C::BN, (M + C::BM - 1) / C::BM);
-    hipLaunchKernelGGL(k_cgemm<T>, grid, dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+    hipLaunchKernelGGL((k_cgemm<T, CFG>), grid, dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
 }
 
-// GEMM with optional per-launch timing, on stream s (default: the context stream).  Counted work:
-// 3M form 6 M N K real flops (FP32), 4M form 8 M N K (FP64); bytes = operands
+template <typename T>
+inline void cgemm_any(int cfg, hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
+                      const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    switch (cfg) {
+        case 1: cgemm_cfg<T, 1>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode); break;
+        case 2: cgemm_cfg<T, 2>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode); break;
+        default: cgemm_cfg<T, 0>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode); break;
+    }
+}
+
+// Blocks of configuration cfg for an M x N product.
+template <typename T>
+inline long gemm_blocks(int cfg, int M, int N) {
+    int bm, bn;
+    switch (cfg) {
+        case 1: bm = GemmCfg<T, 1>::BM; bn = GemmCfg<T, 1>::BN; break;
+        case 2: bm = GemmCfg<T, 2>::BM; bn = GemmCfg<T, 2>::BN; break;
+        default: bm = GemmCfg<T, 0>::BM; bn = GemmCfg<T, 0>::BN; break;
+    }
+    return (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
+}
+
+// The plan's configuration for an M x N product: the large-product tile when
+// it still gives at least `fill` blocks, else the skinny-product tile.
+template <typename T>
+inline int gemm_choose(const bem_ctx* c, int M, int N) {
+    const GemmPlan& p = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
+    return gemm_blocks<T>(p.large, M, N) >= p.fill ? p.large : p.skinny;
+}
+
+// GEMM with optional per-launch timing, on stream s (default: the context
+// stream), tile configuration from the context's plan.  Counted work: 3M
+// form 6 M N K real flops (FP32), 4M form 8 M N K (FP64); bytes = operands
 // read once plus C read and written.
 template <typename T>
 inline void cgemm_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br, const T* Bi,
@@ -235,7 +302,7 @@ inline void cgemm_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai,
     if (M <= 0 || N <= 0 || K <= 0) return;
     if (!s) s = c->stream;
     const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, s);
-    cgemm<T>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+    cgemm_any<T>(gemm_choose<T>(c, M, N), s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
     const double mnk = (double)M * N * K;
     const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
     kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);

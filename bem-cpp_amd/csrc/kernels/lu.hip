@@ -29,10 +29,23 @@
 
 namespace {
 
-constexpr int NB = 256;     // outer panel width = solve block size
-constexpr int PB = 32;      // inner panel width (pivot block)
-constexpr int CH = 256;     // rows per tournament block
-constexpr int G = CH / PB;  // candidate sets merged per block
+constexpr int NB = 256;  // outer panel width = solve block size
+constexpr int PB = 32;   // inner panel width (pivot block)
+
+// Tournament block: CH rows, one per thread, so a merge takes CH / PB
+// candidate sets.  The unrolled elimination needs about 190 registers per
+// thread in FP32 (512 threads fit: three levels up to 131072 rows) and about
+// 300 in FP64 (256 threads).
+template <typename T>
+struct Tslu;
+template <>
+struct Tslu<float> {
+    static constexpr int CH = 512;
+};
+template <>
+struct Tslu<double> {
+    static constexpr int CH = 256;
+};
 
 template <typename T>
 __device__ __forceinline__ T mag2(T r, T i) {
@@ -85,13 +98,14 @@ __device__ __forceinline__ unsigned mag_key(double m) {
     return (unsigned)((unsigned long long)__double_as_longlong(m) >> 32) + 1u;
 }
 
-// Exchange area of the register-resident elimination (double-buffered by
-// step parity so each step needs one barrier).
-template <typename T>
+// Exchange area of the register-resident elimination for NW waves
+// (double-buffered by step parity so each step needs one barrier).
+template <typename T, int NW>
 struct ElimLds {
-    T pr[2][4][PB], pi[2][4][PB];  // best row of each wave
-    unsigned bk[2][4];             // its key
-    int bp[2][4];                  // its thread
+    __align__(16) T pr[2][NW][PB];  // best row of each wave
+    __align__(16) T pi[2][NW][PB];
+    unsigned bk[2][NW];  // its key
+    int bp[2][NW];       // its thread
     int oid[PB], osrc[PB];  // chosen rows in pivot order
     int last;
     int zmin;  // first zero pivot of the root (1-based)
@@ -106,8 +120,10 @@ struct ElimLds {
 // row and eliminates nothing.
 // Afterwards the thread of pivot r holds row r of L11 \ U11 and returns r
 // (-1 for rows not chosen); oid/osrc list the chosen rows in pivot order.
-template <typename T>
-__device__ __forceinline__ int elim_rows(ElimLds<T>& L, T (&xr)[PB], T (&xi)[PB], int id, int src, int nrows) {
+template <typename T, int NW>
+__device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)[PB], int id, int src, int nrows) {
+    using V = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
     const int t = threadIdx.x, wave = t >> 6;
     const bool valid = t < nrows;
     int mine = -1;
@@ -119,12 +135,8 @@ __device__ __forceinline__ int elim_rows(ElimLds<T>& L, T (&xr)[PB], T (&xi)[PB]
         const unsigned wk = __ockl_wfred_max_u32(key);
         const int p = wave * 64 + __ffsll((long long)__ballot(key == wk)) - 1;  // ties: lowest thread
         if (t == p) {
-#pragma unroll
-            for (int c = 0; c < PB; c++)
-                if (c >= jc) {
-                    L.pr[b][wave][c] = xr[c];
-                    L.pi[b][wave][c] = xi[c];
-                }
+            store_row(&L.pr[b][wave][0], xr);
+            store_row(&L.pi[b][wave][0], xi);
             L.bk[b][wave] = wk;
             L.bp[b][wave] = p;
         }
@@ -132,7 +144,7 @@ __device__ __forceinline__ int elim_rows(ElimLds<T>& L, T (&xr)[PB], T (&xi)[PB]
         unsigned best = L.bk[b][0];
         int pp = L.bp[b][0], pw = 0;
 #pragma unroll
-        for (int w = 1; w < 4; w++) {
+        for (int w = 1; w < NW; w++) {
             const unsigned bw = L.bk[b][w];
             if (bw > best) {
                 best = bw;
@@ -143,17 +155,33 @@ __device__ __forceinline__ int elim_rows(ElimLds<T>& L, T (&xr)[PB], T (&xi)[PB]
         if (t == pp) {
             mine = jc;
         } else if (open && best > 1u) {
-            T lr, li;
-            cdiv(xr[jc], xi[jc], L.pr[b][pw][jc], L.pi[b][pw][jc], lr, li);
-            xr[jc] = lr;
-            xi[jc] = li;
+            // the pivot row streamed in 16-byte chunks, starting with the one
+            // holding column jc (which gives the multiplier)
+            const V* vr = reinterpret_cast<const V*>(&L.pr[b][pw][0]);
+            const V* vi = reinterpret_cast<const V*>(&L.pi[b][pw][0]);
+            T lr = T(0), li = T(0);
 #pragma unroll
-            for (int c = 0; c < PB; c++)
-                if (c > jc) {
-                    const T ur = L.pr[b][pw][c], ui = L.pi[b][pw][c];
-                    xr[c] -= lr * ur - li * ui;
-                    xi[c] -= lr * ui + li * ur;
+            for (int q = 0; q < PB / E; q++) {
+                if (q * E + E - 1 < jc) continue;
+                T ur[E], ui[E];
+                const V a = vr[q], cv = vi[q];
+                __builtin_memcpy(ur, &a, 16);
+                __builtin_memcpy(ui, &cv, 16);
+                if (q == jc / E) {
+                    cdiv(xr[jc], xi[jc], ur[jc % E], ui[jc % E], lr, li);
+                    xr[jc] = lr;
+                    xi[jc] = li;
                 }
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int c = q * E + e;
+                    if (c > jc) {
+                        xr[c] -= lr * ur[e] - li * ui[e];
+                        xi[c] -= lr * ui[e] + li * ur[e];
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
     if (mine >= 0) {
@@ -219,11 +247,14 @@ __device__ void root_swaps(const int* __restrict__ oid, int j0, int* __restrict_
 // plane) and the first zero pivot into *info.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_tslu(const T* __restrict__ Ar, const T* __restrict__ Ai, long ld, int j0,
-                                              int n, int* __restrict__ cid, T* __restrict__ cvr, T* __restrict__ cvi,
-                                              int* __restrict__ cnt, int* __restrict__ ipiv, int* __restrict__ pl,
-                                              T* __restrict__ d11r, T* __restrict__ d11i, int* __restrict__ info) {
-    __shared__ ElimLds<T> L;
+__global__ __launch_bounds__(Tslu<T>::CH) void k_tslu(const T* __restrict__ Ar, const T* __restrict__ Ai, long ld,
+                                                      int j0, int n, int* __restrict__ cid, T* __restrict__ cvr,
+                                                      T* __restrict__ cvi, int* __restrict__ cnt,
+                                                      int* __restrict__ ipiv, int* __restrict__ pl,
+                                                      T* __restrict__ d11r, T* __restrict__ d11i,
+                                                      int* __restrict__ info) {
+    constexpr int CH = Tslu<T>::CH, G = CH / PB;
+    __shared__ ElimLds<T, CH / 64> L;
     const int t = threadIdx.x;
     // level 0: rows j0 + b*CH .. (n - j0 is a multiple of PB, so every set is full)
     const int r0 = j0 + blockIdx.x * CH;
@@ -239,7 +270,7 @@ __global__ __launch_bounds__(256) void k_tslu(const T* __restrict__ Ar, const T*
     bool from_matrix = true;
     while (level_sets > 1) {
         // candidate set = original values of the chosen rows
-        for (int e = t; e < PB * PB; e += 256) {
+        for (int e = t; e < PB * PB; e += CH) {
             const int r = e / PB, c = e % PB;
             const long o = ((long)(off + set) * PB + r) * PB + c;
             if (from_matrix) {
@@ -464,10 +495,10 @@ void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<
     const long ld = n;
     for (int j0 = k0; j0 < k0 + NB; j0 += PB) {
         const int m = n - j0;
-        const int nsets = (m + CH - 1) / CH;
+        const int nsets = (m + Tslu<T>::CH - 1) / Tslu<T>::CH;
         int* pl = w.pl + ((j0 - k0) / PB) * PLS;
         int kt = kt_mark(c, KS_TSLU, s);
-        hipLaunchKernelGGL(k_tslu<T>, dim3(nsets), dim3(256), 0, s, Ar, Ai, ld, j0, n, w.cid, w.cvr, w.cvi, w.cnt,
+        hipLaunchKernelGGL(k_tslu<T>, dim3(nsets), dim3(Tslu<T>::CH), 0, s, Ar, Ai, ld, j0, n, w.cid, w.cvr, w.cvi, w.cnt,
                            c->ipiv, pl, w.d11r, w.d11i, w.info);
         kt_close(c, kt, 0.0, 2.0 * sizeof(T) * (double)m * PB);
         launch_laswp<T>(c, s, Ar, Ai, ld, k0, k0 + NB, 0, 0, pl, 1);
@@ -569,7 +600,7 @@ int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     T* Uii = Uir + (size_t)n * NB;
     // work: info, counters, row-pair list, candidate ids, candidate values,
     // diagonal block, U12 buffer
-    const int nsets0 = (n + CH - 1) / CH;
+    const int nsets0 = (n + Tslu<T>::CH - 1) / Tslu<T>::CH;
     const size_t ncnt = 2 * (size_t)nsets0 + 64;
     const size_t nint = 64 + ncnt + (NB / PB) * PLS + 2 * (size_t)nsets0 * PB;
     const size_t bytes = nint * sizeof(int) + 256 + 4 * (size_t)nsets0 * PB * PB * sizeof(T) +
@@ -622,7 +653,8 @@ int lu_factor_launch(bem_ctx* c, int precision) {
     const int n = c->npad;
     const size_t nn = (size_t)n * n;
     hipStream_t s = c->stream;
-    int rc;
+    int rc = gemm_ensure_tuned(c, precision != BEM_F64);
+    if (rc) return rc;
     if (precision == BEM_F64) {
         BEM_HIP(c, hipMemcpyAsync(c->Vr, c->Zr, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
         BEM_HIP(c, hipMemcpyAsync(c->Vi, c->Zi, nn * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -1,0 +1,160 @@
+// tune.hip -- autotuner for the GEMM tile shapes of the factorisation and the
+// solves.  The reference leaves these choices to cuSOLVER; here the LU
+// trailing updates (m x m x 256 products) and the GEMM-based triangular solves
+// and refinement residuals (n x nrhs products) run on the configurations of
+// kernels/gemm.h, and this routine times every configuration at the shapes of
+// the context's system and keeps the fastest per shape class.
+//
+// Every configuration accumulates K in the same order (slices of 16, four
+// MFMA steps of 4), so the choice changes speed only, not results.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "../bem_internal.h"
+#include "gemm.h"
+
+namespace {
+
+template <typename T>
+__global__ void k_fill_tune(T* p, long n, unsigned seed) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned x = (unsigned)i * 2654435761u ^ seed;
+    x ^= x >> 13;
+    x *= 0x5bd1e995u;
+    x ^= x >> 15;
+    p[i] = (T)((x & 0xffffff) * (1.0 / 16777216.0) - 0.5);
+}
+
+struct Shape {
+    int M, N, K;
+};
+
+// Best configuration and its rate (TFLOP/s) for one shape; rates[cfg] filled.
+template <typename T>
+int time_configs(hipStream_t s, const Shape& sh, T* buf, double* rates) {
+    T* Ar = buf;
+    T* Ai = Ar + (size_t)sh.M * sh.K;
+    T* Br = Ai + (size_t)sh.M * sh.K;
+    T* Bi = Br + (size_t)sh.K * sh.N;
+    T* Cr = Bi + (size_t)sh.K * sh.N;
+    T* Ci = Cr + (size_t)sh.M * sh.N;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return -1;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return -1;
+    }
+    const double flops = (sizeof(T) == 4 ? 6.0 : 8.0) * sh.M * (double)sh.N * sh.K;
+    int best = 0;
+    for (int cfg = 0; cfg < kGemmCfgs; cfg++) {
+        auto run = [&]() {
+            cgemm_any<T>(cfg, s, sh.M, sh.N, sh.K, Ar, Ai, sh.K, Br, Bi, sh.N, Cr, Ci, sh.N, 0);
+        };
+        run();  // warm-up (code load, caches)
+        const int reps = 3;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; r++) run();
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        rates[cfg] = ms > 0.0f ? flops * reps / (ms * 1e-3) / 1e12 : 0.0;
+        if (rates[cfg] > rates[best]) best = cfg;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+// Results shared by the contexts of one process: (device, precision, npad, ldr).
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int>, GemmPlan> g_cache;
+
+template <typename T>
+int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* rates_skinny) {
+    GemmPlan& plan = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
+    const auto key = std::make_tuple(c->device, (int)sizeof(T), npad, ldr);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end() && !rates_large) {
+            plan = it->second;  // carries npad
+            return 0;
+        }
+    }
+    // trailing update (capped so the timing stays short) and GEMM-solve shapes
+    const Shape large{std::min(npad, 8192), std::min(npad, 8192), 256};
+    const Shape skinny{npad, ldr, 256};
+    size_t elems = 0;
+    for (const Shape& sh : {large, skinny})
+        elems = std::max(elems, 2 * ((size_t)sh.M * sh.K + (size_t)sh.K * sh.N + (size_t)sh.M * sh.N));
+    T* buf = nullptr;
+    if (hipMalloc(&buf, elems * sizeof(T)) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "autotune buffers");
+    hipLaunchKernelGGL(k_fill_tune<T>, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, c->stream, buf,
+                       (long)elems, 7u);
+    double rl[kGemmCfgs] = {0}, rs[kGemmCfgs] = {0};
+    const int bl = time_configs<T>(c->stream, large, buf, rl);
+    const int bs = time_configs<T>(c->stream, skinny, buf, rs);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess || bl < 0 || bs < 0)
+        return bem_fail(c, BEM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    plan.large = bl;
+    plan.skinny = bs;
+    plan.fill = 2L * cus;  // fewer blocks of the large tile than two per CU: use the skinny tile
+    plan.tuned = true;
+    plan.npad = npad;
+    if (rates_large)
+        for (int i = 0; i < kGemmCfgs; i++) {
+            rates_large[i] = rl[i];
+            rates_skinny[i] = rs[i];
+        }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache[key] = plan;
+    return 0;
+}
+
+}  // namespace
+
+int gemm_autotune_launch(bem_ctx* c, int npad, int nrhs, double* out, int nout) {
+    const int ldr = round_up(std::max(nrhs, 1), 16);
+    double r[4][kGemmCfgs] = {};
+    const bool want = out && nout > 0;
+    if (int rc = tune_precision<float>(c, npad, ldr, want ? r[0] : nullptr, want ? r[1] : nullptr)) return rc;
+    if (int rc = tune_precision<double>(c, npad, ldr, want ? r[2] : nullptr, want ? r[3] : nullptr)) return rc;
+    if (want) {
+        // [precision][shape][cfg] rates, then the chosen (large, skinny) per precision
+        int k = 0;
+        for (int a = 0; a < 4; a++)
+            for (int i = 0; i < kGemmCfgs && k < nout; i++) out[k++] = r[a][i];
+        for (int p = 0; p < 2; p++) {
+            if (k < nout) out[k++] = c->gemm_plan[p].large;
+            if (k < nout) out[k++] = c->gemm_plan[p].skinny;
+        }
+        return k;
+    }
+    return 0;
+}
+
+// Tunes the precision about to be factored once per (device, size) unless the
+// environment sets BEM_AUTOTUNE=0 (then the defaults of GemmPlan stay).
+int gemm_ensure_tuned(bem_ctx* c, bool f32) {
+    GemmPlan& plan = c->gemm_plan[f32 ? 0 : 1];
+    if (plan.tuned && plan.npad == c->npad) return 0;
+    const char* env = std::getenv("BEM_AUTOTUNE");
+    if (env && env[0] == '0') {
+        plan.tuned = true;
+        plan.npad = c->npad;
+        return 0;
+    }
+    const int ldr = round_up(std::max(c->ldr > 0 ? c->ldr : 128, 1), 16);
+    return f32 ? tune_precision<float>(c, c->npad, ldr, nullptr, nullptr)
+               : tune_precision<double>(c, c->npad, ldr, nullptr, nullptr);
+}

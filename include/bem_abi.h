@@ -76,6 +76,17 @@ int bem_timings(const bem_ctx* ctx, double* out, int n);
  * Returns the number of classes. */
 int bem_kernel_stats(bem_ctx* ctx, int mode, double* out, int n);
 
+/* Times the GEMM tile configurations of the factorisation and the solves at
+ * the current system size with nrhs right-hand sides and keeps the fastest
+ * (no counterpart in the reference, which leaves tiling to cuSOLVER; the
+ * factorisation also tunes itself on its first call for a size unless
+ * BEM_AUTOTUNE=0).  out (nullable, nout doubles) receives the measured
+ * TFLOP/s: FP32 trailing-update shape per configuration, FP32 solve shape,
+ * FP64 trailing-update shape, FP64 solve shape (3 configurations each), then
+ * the chosen (large, skinny) configuration per precision.  Returns the
+ * number of values written, or < 0. */
+int bem_autotune(bem_ctx* ctx, int nrhs, double* out, int nout);
+
 /* ---- host-side generators (no device work) ------------------------------ */
 /* Replace icosphere / hex_prism (mesh.h:21-27, mesh.cpp:11-154): shape 0 is
  * the icosphere of radius `param`, shape 1 the hexagonal prism of unit

@@ -122,13 +122,17 @@ def test_mueller_grad_vs_oracle_fd(oracle, bem, shape, ref, param, ka, m, grid, 
 @pytest.mark.gpu
 def test_mueller_grad_c4_vs_oracle_fd_fixture(bem):
     """The adjoint workload: icosphere ref 3 (2N = 3840), ka = 3,
-    m = 1.33 + 0.005i, lab-frame orientation, 181 angles; FD of the FP64 oracle
-    from tests/golden/make_fd_c4.py."""
+    m = 1.33 + 0.005i, lab-frame orientation, 181 angles, central FD (step
+    1e-6) of the FP64 oracle from tests/golden/make_fd_c4.py, for the loss
+    L = sum_t w_t log M11(theta_t) (w uniform in cos theta) and for a loss
+    reaching all 16 elements."""
     from conftest import golden
     g = golden("c4_grad_fd.npz")
     ka, m, nt = float(g["ka"]), complex(g["m"]), int(g["ntheta"])
     # the fixture's own truncation error is far below the tolerance used here
-    assert np.abs(g["fd"] - g["fd_h1e4"]).max() <= 1e-6 * np.abs(g["fd"]).max()
+    for key in ("log", "G"):
+        a, b = g[f"fd_{key}_h1e6"], g[f"fd_{key}_h1e5"]
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
     v, t = bem.make_mesh("sphere", 3, 1.0)
     with bem.Context(0) as ctx:
         ctx.set_mesh(v, t, 7)
@@ -140,6 +144,9 @@ def test_mueller_grad_c4_vs_oracle_fd_fixture(bem):
         ctx.solve(None, ir_tol=1e-12, ir_max=30, return_X=False, nrhs=2)
         M = ctx.orient_mueller(ka, 1.0, RT, w, np.linspace(0, np.pi, nt))
         assert np.abs(M - g["M0"]).max() <= 1e-6 * np.abs(g["M0"][0, 0]).max()
-        grad = ctx.mueller_grad(g["G"])
-    fd = g["fd"]
-    assert np.all(np.abs(grad - fd) <= 1e-5 * np.abs(fd).max()), (grad, fd)
+        G_log = np.zeros((4, 4, nt))
+        G_log[0, 0] = g["w"] / M[0, 0]
+        grad_log = ctx.mueller_grad(G_log)
+        grad_G = ctx.mueller_grad(g["G"])
+    for grad, fd in ((grad_log, g["fd_log_h1e6"]), (grad_G, g["fd_G_h1e6"])):
+        assert np.all(np.abs(grad - fd) <= 1e-5 * np.abs(fd).max()), (grad, fd)

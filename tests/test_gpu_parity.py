@@ -254,3 +254,22 @@ def test_orient_vs_oracle_lossy_hex(oracle, ctx, bem):
     M = ctx.orient_mueller(ka, 1.0, RT, w, th)
     assert _mueller_err(M, Mo) < MUELLER_GATE
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-5
+
+
+def test_autotune_reports_and_factorisation_stays_exact(ctx):
+    # the tuner times every tile configuration at this size; whatever it
+    # picks, the factorisation keeps its backward-error bound
+    rs = np.random.RandomState(8)
+    n = 900
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n)
+    B = rs.randn(n, 16) + 1j * rs.randn(n, 16)
+    ctx.set_Z(Z)
+    rep = ctx.autotune(16)
+    for key in ("fp32_trailing", "fp32_solve", "fp64_trailing", "fp64_solve"):
+        assert all(v > 0 for v in rep[key].values()), rep
+    assert rep["fp32_choice"][0] in ctx.GEMM_TILES and rep["fp64_choice"][1] in ctx.GEMM_TILES64
+    for prec in ("fp64", "fp32"):
+        assert ctx.factor(prec) == 0
+        X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
+        be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
+        assert be < 2e-15
