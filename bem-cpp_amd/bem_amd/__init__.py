@@ -70,7 +70,7 @@ def lib():
     sig = {
         "bem_create": ([i], vp), "bem_destroy": ([vp], None), "bem_last_error": ([vp], ctypes.c_char_p),
         "bem_timings": ([vp, _dp, i], i), "bem_kernel_stats": ([vp, i, _dp, i], i),
-        "bem_autotune": ([vp, i, _dp, i], i),
+        "bem_autotune": ([vp, i, _dp, i], i), "bem_device_copy": ([vp, d], i),
         "bem_mesh_counts": ([i, i, _ip, _ip], i), "bem_make_mesh": ([i, i, d, _dp, _ip], i),
         "bem_orientations": ([i, i, i, i, i, i, i, _dp, _dp], i),
         "bem_set_mesh": ([vp, _dp, i, _ip, i, i], i), "bem_num_rwg": ([vp], i),
@@ -287,6 +287,11 @@ class Context:
                 "fp64_trailing": dict(zip(self.GEMM_TILES64, r[2])), "fp64_solve": dict(zip(self.GEMM_TILES64, r[3])),
                 "fp32_choice": (self.GEMM_TILES[int(out[12])], self.GEMM_TILES[int(out[13])]),
                 "fp64_choice": (self.GEMM_TILES64[int(out[14])], self.GEMM_TILES64[int(out[15])])}
+
+    def device_copy(self, nbytes):
+        """One plain device-to-device copy kernel of nbytes (a known transfer
+        for scaling HBM counters in profiling runs)."""
+        self._check(self._L.bem_device_copy(self._h, float(nbytes)), "device_copy")
 
     KERNEL_CLASSES = ("gemm_fp32_3m", "gemm_fp64", "asm_far", "asm_near", "asm_gather", "tslu", "laswp", "panel",
                       "trsm", "farfield", "rhs", "other")

@@ -34,6 +34,8 @@ void kt_resolve(bem_ctx* c) {
     KernelTimer& k = c->kt;
     if (k.pending.empty()) return;
     (void)hipStreamSynchronize(c->stream2);
+    if (c->stream_upd) (void)hipStreamSynchronize(c->stream_upd);
+    if (c->stream_pnl) (void)hipStreamSynchronize(c->stream_pnl);
     (void)hipStreamSynchronize(c->stream);
     for (KernelTimer::Rec& r : k.pending) {
         float ms = 0.0f;
@@ -207,6 +209,8 @@ void bem_destroy(bem_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream);
+    if (c->stream_upd) (void)hipStreamSynchronize(c->stream_upd);
+    if (c->stream_pnl) (void)hipStreamSynchronize(c->stream_pnl);
     free_mesh(c);
     free_matrices(c);
     free_rhs(c);
@@ -215,6 +219,8 @@ void bem_destroy(bem_ctx* c) {
     (void)hipEventDestroy(c->ev[0]);
     (void)hipEventDestroy(c->ev[1]);
     for (hipEvent_t e : c->lu_ev) (void)hipEventDestroy(e);
+    if (c->stream_upd) (void)hipStreamDestroy(c->stream_upd);
+    if (c->stream_pnl) (void)hipStreamDestroy(c->stream_pnl);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -258,6 +264,12 @@ int bem_autotune(bem_ctx* c, int nrhs, double* out, int nout) {
     (void)hipSetDevice(c->device);
     double local[32];
     return gemm_autotune_launch(c, c->npad, nrhs, out ? out : local, out ? nout : 32);
+}
+
+int bem_device_copy(bem_ctx* c, double bytes) {
+    if (!c || !(bytes >= 16.0) || bytes > 64e9) return bem_fail(c, BEM_ERR_ARG, "device_copy: bad size");
+    (void)hipSetDevice(c->device);
+    return device_copy_launch(c, (size_t)bytes / 16);
 }
 
 int bem_mesh_counts(int shape, int refinements, int* nv, int* nt) {

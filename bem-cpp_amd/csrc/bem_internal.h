@@ -95,6 +95,11 @@ struct bem_ctx {
     KernelTimer kt;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // panel stream of the look-ahead factorisation
+    // factorisation stream layout (see factor() in kernels/lu.hip): 0 one
+    // stream, 1 panel + update streams on all CUs, 2 CU-partitioned
+    int lu_streams = -1;            // -1: not chosen yet
+    int lu_panel_cus = 0;           // CUs reserved for the panel stream (layout 2)
+    hipStream_t stream_upd = nullptr, stream_pnl = nullptr;  // CU-masked streams of layout 2
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t lu_ev[3] = {nullptr, nullptr, nullptr};  // cross-stream ordering inside the factorisation
     std::string err;
@@ -183,6 +188,11 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad);
 // measured TFLOP/s per (precision, shape, configuration).
 int gemm_autotune_launch(bem_ctx* c, int npad, int nrhs, double* out, int nout);
 int gemm_ensure_tuned(bem_ctx* c, bool f32);
+// Chooses the factorisation stream layout (environment BEM_LU_STREAMS,
+// BEM_LU_PANEL_CUS) and creates the CU-masked streams it needs.
+int lu_setup_streams(bem_ctx* c);
+// One copy kernel of n16 16-byte vectors between two fresh device buffers.
+int device_copy_launch(bem_ctx* c, size_t n16);
 // d B / d k_ext of the stored plane waves into (dBr, dBi) (npad x ldr).
 int rhs_dk_launch(bem_ctx* c, double* dBr, double* dBi);
 // Far-field part of the adjoint for the last bem_orient_mueller call: given

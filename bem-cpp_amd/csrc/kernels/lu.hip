@@ -23,6 +23,9 @@
 //   and the rank-256 trailing update A22 -= L21 U12 as matrix-core GEMMs
 //   (gemm.h; 3M form in FP32).  The inverses of all L and U diagonal blocks
 //   are kept for the GEMM-based multi-RHS solves.
+#include <cstdlib>
+#include <vector>
+
 #include "../bem_internal.h"
 #include "gemm.h"
 #include "trsm.h"
@@ -33,14 +36,14 @@ constexpr int NB = 256;  // outer panel width = solve block size
 constexpr int PB = 32;   // inner panel width (pivot block)
 
 // Tournament block: CH rows, one per thread, so a merge takes CH / PB
-// candidate sets.  The unrolled elimination needs about 190 registers per
-// thread in FP32 (512 threads fit: three levels up to 131072 rows) and about
-// 300 in FP64 (256 threads).
+// candidate sets.  256 threads: larger blocks need fewer tree levels but pay
+// more per elimination step (measured slower at 512), and at about 150
+// registers the FP32 block still fits beside the trailing-update GEMM on a CU.
 template <typename T>
 struct Tslu;
 template <>
 struct Tslu<float> {
-    static constexpr int CH = 512;
+    static constexpr int CH = 256;
 };
 template <>
 struct Tslu<double> {
@@ -102,8 +105,9 @@ __device__ __forceinline__ unsigned mag_key(double m) {
 // (double-buffered by step parity so each step needs one barrier).
 template <typename T, int NW>
 struct ElimLds {
-    __align__(16) T pr[2][NW][PB];  // best row of each wave
-    __align__(16) T pi[2][NW][PB];
+    T pr[2][NW][PB];  // best row of each wave (columns >= the step)
+    T pi[2][NW][PB];
+    T rp[2][NW][2];   // 1 / its pivot element
     unsigned bk[2][NW];  // its key
     int bp[2][NW];       // its thread
     int oid[PB], osrc[PB];  // chosen rows in pivot order
@@ -122,8 +126,6 @@ struct ElimLds {
 // (-1 for rows not chosen); oid/osrc list the chosen rows in pivot order.
 template <typename T, int NW>
 __device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)[PB], int id, int src, int nrows) {
-    using V = typename Vec16<T>::type;
-    constexpr int E = 16 / sizeof(T);
     const int t = threadIdx.x, wave = t >> 6;
     const bool valid = t < nrows;
     int mine = -1;
@@ -135,8 +137,17 @@ __device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)
         const unsigned wk = __ockl_wfred_max_u32(key);
         const int p = wave * 64 + __ffsll((long long)__ballot(key == wk)) - 1;  // ties: lowest thread
         if (t == p) {
-            store_row(&L.pr[b][wave][0], xr);
-            store_row(&L.pi[b][wave][0], xi);
+#pragma unroll
+            for (int c = 0; c < PB; c++)
+                if (c > jc) {
+                    L.pr[b][wave][c] = xr[c];
+                    L.pi[b][wave][c] = xi[c];
+                }
+            // reciprocal of the pivot (the rows scale by it, as zgetf2 does)
+            const T d = xr[jc] * xr[jc] + xi[jc] * xi[jc];
+            const T inv = d > T(0) ? T(1) / d : T(0);
+            L.rp[b][wave][0] = xr[jc] * inv;
+            L.rp[b][wave][1] = -xi[jc] * inv;
             L.bk[b][wave] = wk;
             L.bp[b][wave] = p;
         }
@@ -152,37 +163,22 @@ __device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)
                 pw = w;
             }
         }
-        if (t == pp) {
-            mine = jc;
-        } else if (open && best > 1u) {
-            // the pivot row streamed in 16-byte chunks, starting with the one
-            // holding column jc (which gives the multiplier)
-            const V* vr = reinterpret_cast<const V*>(&L.pr[b][pw][0]);
-            const V* vi = reinterpret_cast<const V*>(&L.pi[b][pw][0]);
-            T lr = T(0), li = T(0);
+        // branch-free update: rows that are not eliminated use multiplier 0
+        // (keeps one copy of the row in registers across the step)
+        const bool upd = open && best > 1u && t != pp;
+        if (t == pp) mine = jc;
+        const T qr = L.rp[b][pw][0], qi = L.rp[b][pw][1];
+        const T lr = upd ? xr[jc] * qr - xi[jc] * qi : T(0);
+        const T li = upd ? xr[jc] * qi + xi[jc] * qr : T(0);
+        xr[jc] = upd ? lr : xr[jc];
+        xi[jc] = upd ? li : xi[jc];
 #pragma unroll
-            for (int q = 0; q < PB / E; q++) {
-                if (q * E + E - 1 < jc) continue;
-                T ur[E], ui[E];
-                const V a = vr[q], cv = vi[q];
-                __builtin_memcpy(ur, &a, 16);
-                __builtin_memcpy(ui, &cv, 16);
-                if (q == jc / E) {
-                    cdiv(xr[jc], xi[jc], ur[jc % E], ui[jc % E], lr, li);
-                    xr[jc] = lr;
-                    xi[jc] = li;
-                }
-#pragma unroll
-                for (int e = 0; e < E; e++) {
-                    const int c = q * E + e;
-                    if (c > jc) {
-                        xr[c] -= lr * ur[e] - li * ui[e];
-                        xi[c] -= lr * ui[e] + li * ur[e];
-                    }
-                }
-                __builtin_amdgcn_sched_barrier(0);
+        for (int c = 0; c < PB; c++)
+            if (c > jc) {
+                const T ur = L.pr[b][pw][c], ui = L.pi[b][pw][c];
+                xr[c] -= lr * ur - li * ui;
+                xi[c] -= lr * ui + li * ur;
             }
-        }
     }
     if (mine >= 0) {
         L.oid[mine] = id;
@@ -369,20 +365,29 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict
 
 // Panel step after the swaps, with the factored 32 x 32 diagonal block d11
 // from the tournament.  Block 0 writes d11 in place and forms U12 = L11^{-1}
-// A12 for the nright columns right of the block (one column per thread, in
-// registers).  Blocks >= 1 form L21 = A21 U11^{-1} for 256 rows each (one row
-// per thread, in registers).  A zero pivot leaves its column unscaled.
+// A12 for the nright columns right of the block (one column per thread);
+// blocks >= 1 form L21 = A21 U11^{-1} for 256 rows each (one row per thread).
+// Both substitutions run right-looking with the vector in registers and the
+// triangle streamed from LDS in 16-byte chunks of one row (U) or one column
+// (L, stored transposed), which keeps the kernel near 100 registers so it can
+// share CUs with the trailing-update GEMM.  A zero pivot leaves its column
+// unscaled.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict__ Ai, long ld, int j0, int nrows,
                                                int nright, const T* __restrict__ d11r,
                                                const T* __restrict__ d11i) {
-    __shared__ T ur[PB][PB + 1], ui[PB][PB + 1];
+    using V = typename Vec16<T>::type;
+    constexpr int E = 16 / sizeof(T);
+    __shared__ __align__(16) T ur[PB][PB], ui[PB][PB];  // U11 (rows)
+    __shared__ __align__(16) T lr[PB][PB], li[PB][PB];  // L11 transposed: lr[l][r] = L[r][l]
     __shared__ T rr[PB], ri[PB];
     const int t = threadIdx.x;
     for (int e = t; e < PB * PB; e += 256) {
         const int r = e / PB, c = e % PB;
         ur[r][c] = d11r[e];
         ui[r][c] = d11i[e];
+        lr[c][r] = d11r[e];
+        li[c][r] = d11i[e];
     }
     if (t < PB) {
         const T a = d11r[t * PB + t], b = d11i[t * PB + t];
@@ -394,6 +399,28 @@ __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict
         }
     }
     __syncthreads();
+    // x_c -= s * M[l][c] for c > l, row l of M streamed from LDS
+    auto eliminate = [&](T (&xr)[PB], T (&xi)[PB], const T (*mr)[PB], const T (*mi)[PB], int l, T sr, T si) {
+        const V* vr = reinterpret_cast<const V*>(&mr[l][0]);
+        const V* vi = reinterpret_cast<const V*>(&mi[l][0]);
+#pragma unroll
+        for (int q = 0; q < PB / E; q++) {
+            if (q * E + E - 1 <= l) continue;
+            T a[E], b[E];
+            const V x = vr[q], y = vi[q];
+            __builtin_memcpy(a, &x, 16);
+            __builtin_memcpy(b, &y, 16);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int c = q * E + e;
+                if (c > l) {
+                    xr[c] -= sr * a[e] - si * b[e];
+                    xi[c] -= sr * b[e] + si * a[e];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
     if (blockIdx.x == 0) {
         for (int e = t; e < PB * PB; e += 256) {
             const int r = e / PB, c = e % PB;
@@ -409,16 +436,7 @@ __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict
                 xi[r] = Ai[base + (long)r * ld];
             }
 #pragma unroll
-            for (int r = 1; r < PB; r++) {
-#pragma unroll
-                for (int l = 0; l < PB; l++)
-                    if (l < r) {
-                        const T a = ur[r][l], b = ui[r][l];
-                        xr[r] -= a * xr[l] - b * xi[l];
-                        xi[r] -= a * xi[l] + b * xr[l];
-                    }
-                __builtin_amdgcn_sched_barrier(0);  // keeps the LDS reads of L from being hoisted
-            }
+            for (int l = 0; l < PB - 1; l++) eliminate(xr, xi, lr, li, l, xr[l], xi[l]);  // unit lower
 #pragma unroll
             for (int r = 0; r < PB; r++) {
                 Ar[base + (long)r * ld] = xr[r];
@@ -435,18 +453,11 @@ __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict
     load_row(pr, xr);
     load_row(pi, xi);
 #pragma unroll
-    for (int c = 0; c < PB; c++) {
-        T sr = xr[c], si = xi[c];
-#pragma unroll
-        for (int l = 0; l < PB; l++)
-            if (l < c) {
-                const T a = ur[l][c], b = ui[l][c];
-                sr -= xr[l] * a - xi[l] * b;
-                si -= xr[l] * b + xi[l] * a;
-            }
-        xr[c] = sr * rr[c] - si * ri[c];
-        xi[c] = sr * ri[c] + si * rr[c];
-        __builtin_amdgcn_sched_barrier(0);  // keeps the LDS reads of U from being hoisted
+    for (int l = 0; l < PB; l++) {
+        const T a = xr[l], b = xi[l];
+        xr[l] = a * rr[l] - b * ri[l];  // x_l / U[l][l]
+        xi[l] = a * ri[l] + b * rr[l];
+        if (l < PB - 1) eliminate(xr, xi, ur, ui, l, xr[l], xi[l]);
     }
     store_row(pr, xr);
     store_row(pi, xi);
@@ -517,9 +528,9 @@ void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<
 }
 
 // U12 = L11^{-1} A12 and A22 -= L21 U12 for the columns [c0, c0 + nc) right
-// of outer panel k0, on the context stream; U12 goes through the work buffer.
+// of outer panel k0, on stream us; U12 goes through the work buffer.
 template <typename T>
-void update_columns(bem_ctx* c, T* Ar, T* Ai, const T* Lir, const T* Lii, int k0, int c0, int nc,
+void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, const T* Lii, int k0, int c0, int nc,
                     const LuWork<T>& w) {
     const int n = c->npad;
     const long ld = n;
@@ -528,27 +539,37 @@ void update_columns(bem_ctx* c, T* Ar, T* Ai, const T* Lir, const T* Lii, int k0
     const long o12 = (long)k0 * ld + c0, o21 = (long)(k0 + NB) * ld + k0, o22 = (long)(k0 + NB) * ld + c0;
     T* ubr = w.ubr + (c0 - (k0 + NB));
     T* ubi = w.ubi + (c0 - (k0 + NB));
-    cgemm_ctx<T>(c, NB, nc, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1);
-    cgemm_ctx<T>(c, nr, nc, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0);
-    const int kt = kt_mark(c, KS_OTHER);
-    hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)(((long)NB * nc + 255) / 256)), dim3(256), 0, c->stream, NB,
-                       nc, ubr, ubi, ld, Ar + o12, Ai + o12, ld);
+    cgemm_ctx<T>(c, NB, nc, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1, us);
+    cgemm_ctx<T>(c, nr, nc, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0, us);
+    const int kt = kt_mark(c, KS_OTHER, us);
+    hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)(((long)NB * nc + 255) / 256)), dim3(256), 0, us, NB, nc, ubr,
+                       ubi, ld, Ar + o12, Ai + o12, ld);
     kt_close(c, kt, 0.0, 4.0 * sizeof(T) * (double)NB * nc);
 }
 
 // Blocked right-looking factorisation of the padded npad x npad matrix with
 // look-ahead of depth one: outer panel K+1 is factored on the panel stream
-// while the trailing update of panel K runs on the context stream.  Panel
-// K+1 only touches its own columns; its swaps reach the other columns on the
-// context stream once the update of panel K has finished.
+// while the trailing update of panel K runs on the update stream.  Panel K+1
+// only touches its own columns; its swaps reach the other columns on the
+// update stream once the update of panel K has finished.  Stream layout
+// (c->lu_streams): 0 = everything on the context stream (no overlap);
+// 1 = panel stream and context stream over all CUs; 2 = the panel stream
+// confined to a reserved set of CUs that the update stream does not use, so
+// the latency-bound panel kernels never share a CU with the trailing GEMM.
 template <typename T>
 void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWork<T>& w) {
     const int n = c->npad, nb = n / NB;
     const long ld = n;
     hipStream_t us = c->stream, ps = c->stream2;
+    if (c->lu_streams == 0) ps = us;
+    if (c->lu_streams == 2) {
+        us = c->stream_upd;
+        ps = c->stream_pnl;
+    }
     hipEvent_t e_panel = c->lu_ev[0], e_next = c->lu_ev[1], e_start = c->lu_ev[2];
-    (void)hipEventRecord(e_start, us);  // the matrix and work buffers are ready
+    (void)hipEventRecord(e_start, c->stream);  // the matrix and work buffers are ready
     (void)hipStreamWaitEvent(ps, e_start, 0);
+    if (us != c->stream) (void)hipStreamWaitEvent(us, e_start, 0);
     factor_panel<T>(c, ps, Ar, Ai, 0, w);
     for (int K = 0; K < nb; K++) {
         const int k0 = K * NB;
@@ -558,22 +579,26 @@ void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWo
         launch_laswp<T>(c, us, Ar, Ai, ld, 0, k0, k0 + NB, n, w.pl, NB / PB);
         const long o11 = (long)k0 * ld + k0;
         const long io = (long)K * NB * NB;
-        int kt = kt_mark(c, KS_TRSM);
+        int kt = kt_mark(c, KS_TRSM, us);
         invert_diag_blocks<T>(us, NB, 1, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, true);
         kt_close(c, kt, 8.0 * NB * NB * NB / 6.0, 4.0 * sizeof(T) * (double)NB * NB);
         const int nr = n - (k0 + NB);
         if (nr <= 0) break;
         // columns of panel K+1 first, then hand panel K+1 to the panel stream
-        update_columns<T>(c, Ar, Ai, Lir, Lii, k0, k0 + NB, NB, w);
+        update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + NB, NB, w);
         (void)hipEventRecord(e_next, us);
         (void)hipStreamWaitEvent(ps, e_next, 0);
         factor_panel<T>(c, ps, Ar, Ai, k0 + NB, w);
-        if (nr > NB) update_columns<T>(c, Ar, Ai, Lir, Lii, k0, k0 + 2 * NB, nr - NB, w);
+        if (nr > NB) update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + 2 * NB, nr - NB, w);
     }
     // U^{-1} of all diagonal blocks in one launch
-    const int kt = kt_mark(c, KS_TRSM);
+    const int kt = kt_mark(c, KS_TRSM, us);
     invert_diag_blocks<T>(us, NB, nb, Ar, Ai, ld, Uir, Uii, false);
     kt_close(c, kt, 8.0 * (double)n * NB * NB / 6.0, 4.0 * sizeof(T) * (double)n * NB);
+    if (us != c->stream) {  // the context stream continues after the last update
+        (void)hipEventRecord(e_start, us);
+        (void)hipStreamWaitEvent(c->stream, e_start, 0);
+    }
 }
 
 template <typename T>
@@ -649,12 +674,25 @@ int lu_inverses_launch(bem_ctx* c) {
     return 0;
 }
 
+int lu_setup_streams(bem_ctx* c) {
+    if (c->lu_streams >= 0) return 0;
+    // Layout 2 (CU-masked update and panel streams) is switched off: its first
+    // run ended in a host-side crash inside the factorisation whose cause is
+    // not established, so only the one-stream and the overlapping layouts run.
+    const char* e1 = std::getenv("BEM_LU_STREAMS");
+    const int mode = e1 ? std::atoi(e1) : 1;
+    c->lu_streams = mode == 0 ? 0 : 1;
+    c->lu_panel_cus = 0;
+    return 0;
+}
+
 int lu_factor_launch(bem_ctx* c, int precision) {
     const int n = c->npad;
     const size_t nn = (size_t)n * n;
     hipStream_t s = c->stream;
     int rc = gemm_ensure_tuned(c, precision != BEM_F64);
     if (rc) return rc;
+    if ((rc = lu_setup_streams(c))) return rc;
     if (precision == BEM_F64) {
         BEM_HIP(c, hipMemcpyAsync(c->Vr, c->Zr, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
         BEM_HIP(c, hipMemcpyAsync(c->Vi, c->Zi, nn * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -158,3 +158,30 @@ int gemm_ensure_tuned(bem_ctx* c, bool f32) {
     return f32 ? tune_precision<float>(c, c->npad, ldr, nullptr, nullptr)
                : tune_precision<double>(c, c->npad, ldr, nullptr, nullptr);
 }
+
+namespace {
+__global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+}  // namespace
+
+int device_copy_launch(bem_ctx* c, size_t n16) {
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n16 * 16) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "device_copy: source");
+    if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return bem_fail(c, BEM_ERR_NOMEM, "device_copy: destination");
+    }
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    hipError_t e = hipMemsetAsync(a, 0, n16 * 16, c->stream);
+    if (!e) {
+        hipLaunchKernelGGL(k_copy16, dim3(8 * cus), dim3(256), 0, c->stream, a, b, n16);
+        e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return bem_fail(c, BEM_ERR_HIP, std::string("device_copy: ") + hipGetErrorString(e));
+    return 0;
+}

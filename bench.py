@@ -48,6 +48,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+_EXTERNAL_LU = r"""
+import json, sys, torch
+n = int(sys.argv[1])
+torch.backends.cuda.preferred_linalg_library("cusolver")
+g = torch.Generator(device="cuda:0")
+g.manual_seed(42)
+A = torch.randn(n, n, dtype=torch.complex128, device="cuda:0", generator=g)
+ms = []
+for _ in range(2):
+    B = A.clone()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    torch.linalg.lu_factor(B)
+    e1.record()
+    torch.cuda.synchronize()
+    ms.append(e0.elapsed_time(e1))
+print(json.dumps({"factor_ms": ms[-1]}))
+"""
+
+
+def external_lu(n):
+    """torch.linalg.lu_factor (hipSOLVER -> rocSOLVER zgetrf) on a random
+    complex128 n x n matrix: the library FP64 LU that the reference's cuSOLVER
+    Zgetrf corresponds to, timed in a child process (torch's HIP runtime and
+    this library's cannot share one process).  ms of the second call."""
+    import subprocess
+    lib = "torch.linalg.lu_factor (hipSOLVER/rocSOLVER zgetrf), complex128, same GPU"
+    try:
+        r = subprocess.run([sys.executable, "-c", _EXTERNAL_LU, str(n)], capture_output=True, text=True, timeout=240)
+        if r.returncode != 0:
+            return {"library": lib, "error": (r.stderr or r.stdout)[-300:]}
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+        return {"library": lib, "n": n, "factor_ms": out["factor_ms"]}
+    except Exception as exc:
+        return {"library": lib, "error": str(exc)[:300]}
+
+
 def measured_traffic(workload, kernel_class):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of
@@ -109,6 +147,8 @@ def main():
     ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-external", action="store_true",
+                    help="skip timing the library FP64 LU (torch.linalg.lu_factor on rocSOLVER)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the steps, copy MIB MiB device-to-device with one torch kernel (a known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters of profiling runs)")
@@ -131,15 +171,8 @@ def main():
 
     ctx = bem_amd.Context(0)
     if args.copy_calibration > 0:
-        import torch
-        nel = args.copy_calibration * (1 << 20) // 4
-        src = torch.ones(nel, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy calibration: {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
-        del src, dst
+        ctx.device_copy(args.copy_calibration * (1 << 20))
+        log(f"copy calibration (k_copy16): {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
     N = ctx.set_mesh(verts, tris, 7)
     n = 2 * N
     log(f"workload {args.workload}: {shape} ref {ref} T={len(tris)} N={N} 2N={n} ka={ka} m={m} "
@@ -193,6 +226,15 @@ def main():
                     mueller_err_fp32ir_vs_fp64=mueller_err(M32, M64))
         log(f"fp64: {fp64}")
 
+    # GEMM tile plan picked by the autotuner at this size (timed again here, outside the steps)
+    tune = ctx.autotune(nrhs)
+
+    # library FP64 LU of the same size on the same GPU (external reference)
+    external = None
+    if not args.skip_external:
+        external = external_lu(n)
+        log(f"external FP64 LU: {external}")
+
     # roofline of the dominant kernel class (live HIP-event times over the timed steps)
     dom = max(kstats, key=lambda k: kstats[k][1])
     launches, kms, kflops, kbytes = kstats[dom]
@@ -231,6 +273,9 @@ def main():
             "n": n, "nrhs": nrhs, "n_orient": n_orient, "ntheta": ntheta,
         },
         "z_entries_per_s": n * n / (ph["assembly"] * 1e-3),
+        "gemm_plan": {"fp32": tune["fp32_choice"], "fp64": tune["fp64_choice"],
+                      "fp32_trailing_tflops": tune["fp32_trailing"], "fp64_trailing_tflops": tune["fp64_trailing"]},
+        "external_fp64_lu": external,
         "s_per_orient_fp32ir_excl_assembly": s_orient_excl_asm,
         "s_per_orient_fp64_excl_assembly": fp64.get("s_per_orient"),
         "speedup_fp32ir_vs_fp64": (fp64["s_per_orient"] / s_orient_excl_asm) if fp64 else None,

@@ -87,6 +87,12 @@ int bem_kernel_stats(bem_ctx* ctx, int mode, double* out, int n);
  * number of values written, or < 0. */
 int bem_autotune(bem_ctx* ctx, int nrhs, double* out, int nout);
 
+/* Copies `bytes` bytes (rounded down to 16) device to device with one plain
+ * vectorised kernel: a transfer of known size, the scale for the HBM
+ * FETCH_SIZE / WRITE_SIZE counters of profiling runs (no counterpart in the
+ * reference).  Returns 0 or < 0. */
+int bem_device_copy(bem_ctx* ctx, double bytes);
+
 /* ---- host-side generators (no device work) ------------------------------ */
 /* Replace icosphere / hex_prism (mesh.h:21-27, mesh.cpp:11-154): shape 0 is
  * the icosphere of radius `param`, shape 1 the hexagonal prism of unit
