@@ -71,6 +71,7 @@ def lib():
         "bem_create": ([i], vp), "bem_destroy": ([vp], None), "bem_last_error": ([vp], ctypes.c_char_p),
         "bem_timings": ([vp, _dp, i], i), "bem_kernel_stats": ([vp, i, _dp, i], i),
         "bem_autotune": ([vp, i, _dp, i], i), "bem_device_copy": ([vp, d], i),
+        "bem_gemm_plan": ([vp, i, i, i, ctypes.c_long], i), "bem_gemm_plan_get": ([vp, i, _dp, i], i),
         "bem_mesh_counts": ([i, i, _ip, _ip], i), "bem_make_mesh": ([i, i, d, _dp, _ip], i),
         "bem_orientations": ([i, i, i, i, i, i, i, _dp, _dp], i),
         "bem_set_mesh": ([vp, _dp, i, _ip, i, i], i), "bem_num_rwg": ([vp], i),
@@ -273,20 +274,40 @@ class Context:
         self._check(self._L.bem_mueller_grad(self._h, _p(d), _p(g)), "mueller_grad")
         return g
 
-    GEMM_TILES = ("128x128", "128x64", "64x64")  # FP32 configurations
-    GEMM_TILES64 = ("128x64", "64x64", "64x32")  # FP64 configurations
+    # GEMM configurations (block tile; "k8": K slice of 8 instead of 16)
+    GEMM_TILES = ("128x128", "128x64", "64x64", "64x64k8")  # FP32
+    GEMM_TILES64 = ("128x64", "64x64", "64x32", "64x64k8")  # FP64
 
     def autotune(self, nrhs=128):
         """Times the GEMM tile configurations at this system's shapes and keeps
         the fastest; returns the measured TFLOP/s and the choices."""
-        out = np.zeros(16)
-        n = self._L.bem_autotune(self._h, int(nrhs), _p(out), 16)
+        nc = len(self.GEMM_TILES)
+        out = np.zeros(4 * nc + 4)
+        n = self._L.bem_autotune(self._h, int(nrhs), _p(out), len(out))
         self._check(n, "autotune")
-        r = out[:12].reshape(4, 3)
+        r = out[:4 * nc].reshape(4, nc)
         return {"fp32_trailing": dict(zip(self.GEMM_TILES, r[0])), "fp32_solve": dict(zip(self.GEMM_TILES, r[1])),
                 "fp64_trailing": dict(zip(self.GEMM_TILES64, r[2])), "fp64_solve": dict(zip(self.GEMM_TILES64, r[3])),
-                "fp32_choice": (self.GEMM_TILES[int(out[12])], self.GEMM_TILES[int(out[13])]),
-                "fp64_choice": (self.GEMM_TILES64[int(out[14])], self.GEMM_TILES64[int(out[15])])}
+                "fp32_choice": (self.GEMM_TILES[int(out[4 * nc])], self.GEMM_TILES[int(out[4 * nc + 1])]),
+                "fp64_choice": (self.GEMM_TILES64[int(out[4 * nc + 2])], self.GEMM_TILES64[int(out[4 * nc + 3])])}
+
+    def gemm_plan(self, prec, large, skinny, cap=0):
+        """Fixes the GEMM configurations (indices into GEMM_TILES /
+        GEMM_TILES64) of one precision and the grid cap of the capped trailing
+        updates (0: automatic), replacing the autotuner's choice."""
+        p = {"fp32": 0, "fp64": 1}[prec]
+        self._check(self._L.bem_gemm_plan(self._h, p, int(large), int(skinny), int(cap)), "gemm_plan")
+
+    def gemm_plan_info(self, prec):
+        """The current GEMM plan of one precision: tile names, grid cap, and
+        the trial factorisation time per configuration (ms; 0 = not run)."""
+        tiles = self.GEMM_TILES if prec == "fp32" else self.GEMM_TILES64
+        out = np.zeros(6 + len(tiles))
+        self._check(self._L.bem_gemm_plan_get(self._h, {"fp32": 0, "fp64": 1}[prec], _p(out), len(out)),
+                    "gemm_plan_get")
+        return {"large": tiles[int(out[0])], "skinny": tiles[int(out[1])], "panel": tiles[int(out[2])],
+                "cap": int(out[3]), "tuned": bool(out[4]), "trialed": bool(out[5]),
+                "trial_factor_ms": {t: float(v) for t, v in zip(tiles, out[6:]) if v > 0}}
 
     def device_copy(self, nbytes):
         """One plain device-to-device copy kernel of nbytes (a known transfer

@@ -193,8 +193,9 @@ bem_ctx* bem_create(int device) {
         hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
         hipEventCreateWithFlags(&c->lu_ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->lu_ev[1], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->lu_ev[2], hipEventDisableTiming) != hipSuccess) {
-        for (hipEvent_t e : {c->ev[0], c->ev[1], c->lu_ev[0], c->lu_ev[1], c->lu_ev[2]})
+        hipEventCreateWithFlags(&c->lu_ev[2], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->lu_ev[3], hipEventDisableTiming) != hipSuccess) {
+        for (hipEvent_t e : {c->ev[0], c->ev[1], c->lu_ev[0], c->lu_ev[1], c->lu_ev[2], c->lu_ev[3]})
             if (e) (void)hipEventDestroy(e);
         for (hipStream_t s : {c->stream, c->stream2})
             if (s) (void)hipStreamDestroy(s);
@@ -264,6 +265,29 @@ int bem_autotune(bem_ctx* c, int nrhs, double* out, int nout) {
     (void)hipSetDevice(c->device);
     double local[32];
     return gemm_autotune_launch(c, c->npad, nrhs, out ? out : local, out ? nout : 32);
+}
+
+int bem_gemm_plan(bem_ctx* c, int prec, int large, int skinny, long cap) {
+    if (!c) return BEM_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    return gemm_set_plan(c, prec, large, skinny, cap);
+}
+
+int bem_gemm_plan_get(bem_ctx* c, int prec, double* out, int nout) {
+    if (!c || prec < 0 || prec > 1 || !out || nout < 0) return bem_fail(c, BEM_ERR_ARG, "gemm_plan_get: bad arguments");
+    const GemmPlan& p = c->gemm_plan[prec];
+    double v[8 + GemmPlan::kMaxCfgs];
+    int k = 0;
+    v[k++] = p.large;
+    v[k++] = p.skinny;
+    v[k++] = p.panel;
+    v[k++] = (double)p.cap;
+    v[k++] = p.tuned ? 1.0 : 0.0;
+    v[k++] = p.trialed ? 1.0 : 0.0;
+    for (double t : p.trial_ms) v[k++] = t;
+    const int m = std::min(k, nout);
+    for (int i = 0; i < m; i++) out[i] = v[i];
+    return m;
 }
 
 int bem_device_copy(bem_ctx* c, double bytes) {

@@ -81,10 +81,17 @@ struct KernelTimer {
 // GEMM tile choice of one precision (configurations of kernels/gemm.h):
 // `large` for products with at least `fill` blocks in it, else `skinny`.
 struct GemmPlan {
+    static constexpr int kMaxCfgs = 4;
     int large = 0, skinny = 2;
+    int panel = 3;                           // products of the panel stream (smallest LDS footprint)
     long fill = 512;
     bool tuned = false;
-    int npad = 0;  // system size the plan was tuned for
+    int npad = 0;                            // system size the plan was tuned for
+    int cus = 256;                           // compute units of the device
+    int occ[kMaxCfgs] = {1, 1, 1, 1};        // resident blocks per CU of each configuration
+    long cap = 0;                            // grid of capped launches; 0: from occ and cus
+    bool trialed = false;                    // `large` chosen by whole trial factorisations
+    double trial_ms[kMaxCfgs] = {0, 0, 0, 0};  // their times per configuration (0: not run)
     GemmPlan() = default;
     GemmPlan(int l, int s) : large(l), skinny(s) {}
 };
@@ -101,7 +108,7 @@ struct bem_ctx {
     int lu_panel_cus = 0;           // CUs reserved for the panel stream (layout 2)
     hipStream_t stream_upd = nullptr, stream_pnl = nullptr;  // CU-masked streams of layout 2
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t lu_ev[3] = {nullptr, nullptr, nullptr};  // cross-stream ordering inside the factorisation
+    hipEvent_t lu_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // cross-stream ordering inside the factorisation
     std::string err;
 
     // mesh and basis (host copies)
@@ -188,6 +195,13 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad);
 // measured TFLOP/s per (precision, shape, configuration).
 int gemm_autotune_launch(bem_ctx* c, int npad, int nrhs, double* out, int nout);
 int gemm_ensure_tuned(bem_ctx* c, bool f32);
+// Sets the plan of one precision (0 FP32, 1 FP64) to the given configurations
+// and grid cap of the capped trailing updates (0: one block short of full
+// occupancy per CU), replacing the autotuner's choice.
+int gemm_set_plan(bem_ctx* c, int prec, int large, int skinny, long cap);
+// Stores the context's plan of one precision in the per-process cache of
+// tuned plans (after the factorisation trials changed it).
+void gemm_cache_store(bem_ctx* c, bool f32);
 // Chooses the factorisation stream layout (environment BEM_LU_STREAMS,
 // BEM_LU_PANEL_CUS) and creates the CU-masked streams it needs.
 int lu_setup_streams(bem_ctx* c);

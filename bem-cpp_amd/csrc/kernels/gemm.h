@@ -1,7 +1,14 @@
 // gemm.h -- complex GEMM on CDNA4 matrix cores, planar operands.
 //
 //   C = C - A B   (mode 0)     C = A B   (mode 1)     C = C + A B   (mode 2)
+//   C = -A B      (mode 3)
 //
+// mode | GEMM_MODE_URGENT raises the waves' issue priority (s_setprio), for
+// the small products on the factorisation's critical path that share CUs
+// with a running trailing update.
+//
+// A batch of independent products of one shape runs in one launch
+// (blockIdx.z), operand z at the base pointers plus z times the batch strides.
 // A is M x K, B is K x N, C is M x N; every matrix is stored as two planes
 // (real, imaginary) in row-major order with its own leading dimension.
 //
@@ -20,6 +27,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+constexpr int GEMM_MODE_URGENT = 4;
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -81,7 +90,23 @@ struct GemmCfg<double, 2> {
     typedef double2 vec_t;
     static constexpr int VEC = 2;
 };
-constexpr int kGemmCfgs = 3;
+// 64 x 64 with a K slice of 8: half the LDS of configuration 2 / 1, so more
+// blocks fit on a CU beside the latency-bound panel kernels.
+template <>
+struct GemmCfg<float, 3> {
+    static constexpr int BM = 64, BN = 64, BK = 8, WM = 32, WN = 32, PAD = 16, MINB = 2;
+    typedef v4f acc_t;
+    typedef float4 vec_t;
+    static constexpr int VEC = 4;
+};
+template <>
+struct GemmCfg<double, 3> {
+    static constexpr int BM = 64, BN = 64, BK = 8, WM = 32, WN = 32, PAD = 8, MINB = 2;
+    typedef v4d acc_t;
+    typedef double2 vec_t;
+    static constexpr int VEC = 2;
+};
+constexpr int kGemmCfgs = 4;
 
 template <typename T>
 __device__ __forceinline__ void vec_split(const typename GemmCfg<T, 0>::vec_t& v, T* out);
@@ -98,178 +123,235 @@ template <typename T, int CFG>
 __global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, int N, int K, const T* __restrict__ Ar,
                                                const T* __restrict__ Ai, long lda, const T* __restrict__ Br,
                                                const T* __restrict__ Bi, long ldb, T* __restrict__ Cr,
-                                               T* __restrict__ Ci, long ldc, int mode) {
+                                               T* __restrict__ Ci, long ldc, int mode, int nz, long sa, long sb,
+                                               long sc) {
     typedef GemmCfg<T, CFG> C;
     constexpr int BM = C::BM, BN = C::BN, BK = C::BK, WM = C::WM, WN = C::WN, VEC = C::VEC;
     constexpr int TM = WM / 16, TN = WN / 16;
     constexpr int SA = BM + C::PAD, SB = BN + C::PAD;
-    constexpr int AV = BM * BK / VEC / 256;  // 16-byte vectors per thread per plane
-    constexpr int BV = BK * BN / VEC / 256;
+    constexpr int AVT = BM * BK / VEC, BVT = BK * BN / VEC;  // 16-byte vectors per plane per slice
+    constexpr int AV = (AVT + 255) / 256, BV = (BVT + 255) / 256;  // per thread
     typedef typename C::vec_t vec_t;
     typedef typename C::acc_t acc_t;
+    constexpr int NACC = sizeof(T) == 4 ? 3 : 2;
 
     __shared__ T sA[2][2][BK][SA];
     __shared__ T sB[2][2][BK][SB];
 
+    if (mode & GEMM_MODE_URGENT) __builtin_amdgcn_s_setprio(3);
+    mode &= 3;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
     const int wm0 = (wave >> 1) * WM, wn0 = (wave & 1) * WN;
+    const int ntx = (N + BN - 1) / BN, nty = (M + BM - 1) / BM;
+    const int ntiles = ntx * nty * nz;
+    const int nk = K / BK;
 
-    vec_t ga[2][AV], gb[2][BV];
-    auto load = [&](int k0) {
+    // Tiles in row-major order over (z, row, column); a grid smaller than the
+    // tile count makes every block loop (persistent launch).
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int bz = tile / (ntx * nty), rem = tile - bz * (ntx * nty);
+        const int row0 = (rem / ntx) * BM, col0 = (rem % ntx) * BN;
+        const T* __restrict__ ar_ = Ar + bz * sa;
+        const T* __restrict__ ai_ = Ai + bz * sa;
+        const T* __restrict__ br_ = Br + bz * sb;
+        const T* __restrict__ bi_ = Bi + bz * sb;
+
+        vec_t ga[2][AV], gb[2][BV];
+        auto load = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < AV; u++) {
-            const int v = tid + 256 * u;
-            const int r = v / (BK / VEC), kv = (v % (BK / VEC)) * VEC;
-            const bool ok = row0 + r < M;
-            const long o = (long)(row0 + r) * lda + k0 + kv;
-            ga[0][u] = ok ? *reinterpret_cast<const vec_t*>(Ar + o) : vec_t{};
-            ga[1][u] = ok ? *reinterpret_cast<const vec_t*>(Ai + o) : vec_t{};
-        }
-#pragma unroll
-        for (int u = 0; u < BV; u++) {
-            const int v = tid + 256 * u;
-            const int kr = v / (BN / VEC), nv = (v % (BN / VEC)) * VEC;
-            const bool ok = col0 + nv < N;
-            const long o = (long)(k0 + kr) * ldb + col0 + nv;
-            gb[0][u] = ok ? *reinterpret_cast<const vec_t*>(Br + o) : vec_t{};
-            gb[1][u] = ok ? *reinterpret_cast<const vec_t*>(Bi + o) : vec_t{};
-        }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < AV; u++) {
-            const int v = tid + 256 * u;
-            const int r = v / (BK / VEC), kv = (v % (BK / VEC)) * VEC;
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-                T e[VEC];
-                vec_split<T>(ga[p][u], e);
-#pragma unroll
-                for (int q = 0; q < VEC; q++) sA[buf][p][kv + q][r] = e[q];
+            for (int u = 0; u < AV; u++) {
+                const int v = tid + 256 * u;
+                const int r = v / (BK / VEC), kv = (v % (BK / VEC)) * VEC;
+                const bool ok = (AVT % 256 == 0 || v < AVT) && row0 + r < M;
+                const long o = (long)(row0 + r) * lda + k0 + kv;
+                ga[0][u] = ok ? *reinterpret_cast<const vec_t*>(ar_ + o) : vec_t{};
+                ga[1][u] = ok ? *reinterpret_cast<const vec_t*>(ai_ + o) : vec_t{};
             }
-        }
 #pragma unroll
-        for (int u = 0; u < BV; u++) {
-            const int v = tid + 256 * u;
-            const int kr = v / (BN / VEC), nv = (v % (BN / VEC)) * VEC;
+            for (int u = 0; u < BV; u++) {
+                const int v = tid + 256 * u;
+                const int kr = v / (BN / VEC), nv = (v % (BN / VEC)) * VEC;
+                const bool ok = (BVT % 256 == 0 || v < BVT) && col0 + nv < N;
+                const long o = (long)(k0 + kr) * ldb + col0 + nv;
+                gb[0][u] = ok ? *reinterpret_cast<const vec_t*>(br_ + o) : vec_t{};
+                gb[1][u] = ok ? *reinterpret_cast<const vec_t*>(bi_ + o) : vec_t{};
+            }
+        };
+        auto store = [&](int buf) {
 #pragma unroll
-            for (int p = 0; p < 2; p++) *reinterpret_cast<vec_t*>(&sB[buf][p][kr][nv]) = gb[p][u];
-        }
-    };
+            for (int u = 0; u < AV; u++) {
+                const int v = tid + 256 * u;
+                if (AVT % 256 != 0 && v >= AVT) continue;
+                const int r = v / (BK / VEC), kv = (v % (BK / VEC)) * VEC;
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    T e[VEC];
+                    vec_split<T>(ga[p][u], e);
+#pragma unroll
+                    for (int q = 0; q < VEC; q++) sA[buf][p][kv + q][r] = e[q];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < BV; u++) {
+                const int v = tid + 256 * u;
+                if (BVT % 256 != 0 && v >= BVT) continue;
+                const int kr = v / (BN / VEC), nv = (v % (BN / VEC)) * VEC;
+#pragma unroll
+                for (int p = 0; p < 2; p++) *reinterpret_cast<vec_t*>(&sB[buf][p][kr][nv]) = gb[p][u];
+            }
+        };
 
-    constexpr int NACC = sizeof(T) == 4 ? 3 : 2;
-    acc_t acc[NACC][TM][TN];
+        acc_t acc[NACC][TM][TN];
 #pragma unroll
-    for (int a = 0; a < NACC; a++)
+        for (int a = 0; a < NACC; a++)
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) acc[a][i][j] = acc_t{};
+
+        load(0);
+        store(0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; kt++) {
+            const int buf = kt & 1;
+            if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                const int ka = kk + (lane >> 4);
+                T ar[TM], ai[TM], br[TN], bi[TN];
+#pragma unroll
+                for (int i = 0; i < TM; i++) {
+                    ar[i] = sA[buf][0][ka][wm0 + i * 16 + (lane & 15)];
+                    ai[i] = sA[buf][1][ka][wm0 + i * 16 + (lane & 15)];
+                }
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    br[j] = sB[buf][0][ka][wn0 + j * 16 + (lane & 15)];
+                    bi[j] = sB[buf][1][ka][wn0 + j * 16 + (lane & 15)];
+                }
+                if constexpr (sizeof(T) == 4) {
+                    T as[TM], bs[TN];
+#pragma unroll
+                    for (int i = 0; i < TM; i++) as[i] = ar[i] + ai[i];
+#pragma unroll
+                    for (int j = 0; j < TN; j++) bs[j] = br[j] + bi[j];
+#pragma unroll
+                    for (int i = 0; i < TM; i++)
+#pragma unroll
+                        for (int j = 0; j < TN; j++) {
+                            acc[0][i][j] = mfma16x16x4(ar[i], br[j], acc[0][i][j]);
+                            acc[1][i][j] = mfma16x16x4(ai[i], bi[j], acc[1][i][j]);
+                            acc[2][i][j] = mfma16x16x4(as[i], bs[j], acc[2][i][j]);
+                        }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < TM; i++)
+#pragma unroll
+                        for (int j = 0; j < TN; j++) {
+                            acc[0][i][j] = mfma16x16x4(ar[i], br[j], acc[0][i][j]);
+                            acc[0][i][j] = mfma16x16x4(ai[i], -bi[j], acc[0][i][j]);
+                            acc[1][i][j] = mfma16x16x4(ar[i], bi[j], acc[1][i][j]);
+                            acc[1][i][j] = mfma16x16x4(ai[i], br[j], acc[1][i][j]);
+                        }
+                }
+            }
+            if (kt + 1 < nk) store(buf ^ 1);
+            __syncthreads();  // also ends the reads of this tile before the next tile's first store
+        }
+
+        T* __restrict__ cr_ = Cr + bz * sc;
+        T* __restrict__ ci_ = Ci + bz * sc;
 #pragma unroll
         for (int i = 0; i < TM; i++)
 #pragma unroll
-            for (int j = 0; j < TN; j++) acc[a][i][j] = acc_t{};
-
-    const int nk = K / BK;
-    load(0);
-    store(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; kt++) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) load((kt + 1) * BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            const int ka = kk + (lane >> 4);
-            T ar[TM], ai[TM], br[TN], bi[TN];
-#pragma unroll
-            for (int i = 0; i < TM; i++) {
-                ar[i] = sA[buf][0][ka][wm0 + i * 16 + (lane & 15)];
-                ai[i] = sA[buf][1][ka][wm0 + i * 16 + (lane & 15)];
-            }
-#pragma unroll
             for (int j = 0; j < TN; j++) {
-                br[j] = sB[buf][0][ka][wn0 + j * 16 + (lane & 15)];
-                bi[j] = sB[buf][1][ka][wn0 + j * 16 + (lane & 15)];
-            }
-            if constexpr (sizeof(T) == 4) {
-                T as[TM], bs[TN];
+                const int col = col0 + wn0 + j * 16 + (lane & 15);
 #pragma unroll
-                for (int i = 0; i < TM; i++) as[i] = ar[i] + ai[i];
-#pragma unroll
-                for (int j = 0; j < TN; j++) bs[j] = br[j] + bi[j];
-#pragma unroll
-                for (int i = 0; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++) {
-                        acc[0][i][j] = mfma16x16x4(ar[i], br[j], acc[0][i][j]);
-                        acc[1][i][j] = mfma16x16x4(ai[i], bi[j], acc[1][i][j]);
-                        acc[2][i][j] = mfma16x16x4(as[i], bs[j], acc[2][i][j]);
+                for (int r = 0; r < 4; r++) {
+                    const int row = row0 + wm0 + i * 16 + (sizeof(T) == 4 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r);
+                    if (row >= M || col >= N) continue;
+                    T vr, vi;
+                    if constexpr (sizeof(T) == 4) {
+                        vr = acc[0][i][j][r] - acc[1][i][j][r];
+                        vi = acc[2][i][j][r] - acc[0][i][j][r] - acc[1][i][j][r];
+                    } else {
+                        vr = acc[0][i][j][r];
+                        vi = acc[1][i][j][r];
                     }
-            } else {
-#pragma unroll
-                for (int i = 0; i < TM; i++)
-#pragma unroll
-                    for (int j = 0; j < TN; j++) {
-                        acc[0][i][j] = mfma16x16x4(ar[i], br[j], acc[0][i][j]);
-                        acc[0][i][j] = mfma16x16x4(ai[i], -bi[j], acc[0][i][j]);
-                        acc[1][i][j] = mfma16x16x4(ar[i], bi[j], acc[1][i][j]);
-                        acc[1][i][j] = mfma16x16x4(ai[i], br[j], acc[1][i][j]);
+                    const long o = (long)row * ldc + col;
+                    if (mode == 0) {
+                        cr_[o] -= vr;
+                        ci_[o] -= vi;
+                    } else if (mode == 1) {
+                        cr_[o] = vr;
+                        ci_[o] = vi;
+                    } else if (mode == 3) {
+                        cr_[o] = -vr;
+                        ci_[o] = -vi;
+                    } else {
+                        cr_[o] += vr;
+                        ci_[o] += vi;
                     }
+                }
             }
-        }
-        if (kt + 1 < nk) store(buf ^ 1);
-        __syncthreads();
     }
-
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-            const int col = col0 + wn0 + j * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = row0 + wm0 + i * 16 + (sizeof(T) == 4 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r);
-                if (row >= M || col >= N) continue;
-                T vr, vi;
-                if constexpr (sizeof(T) == 4) {
-                    vr = acc[0][i][j][r] - acc[1][i][j][r];
-                    vi = acc[2][i][j][r] - acc[0][i][j][r] - acc[1][i][j][r];
-                } else {
-                    vr = acc[0][i][j][r];
-                    vi = acc[1][i][j][r];
-                }
-                const long o = (long)row * ldc + col;
-                if (mode == 0) {
-                    Cr[o] -= vr;
-                    Ci[o] -= vi;
-                } else if (mode == 1) {
-                    Cr[o] = vr;
-                    Ci[o] = vi;
-                } else {
-                    Cr[o] += vr;
-                    Ci[o] += vi;
-                }
-            }
-        }
 }
 
-// Host launcher of configuration cfg.  K must be a multiple of BK (16); N a
+// Host launcher of configuration cfg.  K must be a multiple of BK; N a
 // multiple of the vector width (4 floats / 2 doubles) and every leading
-// dimension a multiple of it.
+// dimension a multiple of it.  nb products with batch strides sa, sb, sc.
+// cap > 0 limits the grid to cap blocks, which then loop over the tiles.
 template <typename T, int CFG>
 inline void cgemm_cfg(hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
-                      const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
+                      const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode, int nb, long sa, long sb, long sc,
+                      long cap) {
     typedef GemmCfg<T, CFG> C;
-    dim3 grid((N + C::BN - 1) / C::BN, (M + C::BM - 1) / C::BM);
-    hipLaunchKernelGGL((k_cgemm<T, CFG>), grid, dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+    const long tiles = (long)((N + C::BN - 1) / C::BN) * ((M + C::BM - 1) / C::BM) * nb;
+    const long grid = cap > 0 && cap < tiles ? cap : tiles;
+    hipLaunchKernelGGL((k_cgemm<T, CFG>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr,
+                       Ci, ldc, mode, nb, sa, sb, sc);
 }
 
 template <typename T>
 inline void cgemm_any(int cfg, hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
-                      const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
-    if (M <= 0 || N <= 0 || K <= 0) return;
+                      const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode, int nb = 1, long sa = 0, long sb = 0,
+                      long sc = 0, long cap = 0) {
+    if (M <= 0 || N <= 0 || K <= 0 || nb <= 0) return;
     switch (cfg) {
-        case 1: cgemm_cfg<T, 1>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode); break;
-        case 2: cgemm_cfg<T, 2>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode); break;
-        default: cgemm_cfg<T, 0>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode); break;
+        case 1: cgemm_cfg<T, 1>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
+        case 2: cgemm_cfg<T, 2>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
+        case 3: cgemm_cfg<T, 3>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
+        default: cgemm_cfg<T, 0>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
     }
+}
+
+// Blocks of configuration cfg that fit on one CU (registers, LDS, waves).
+template <typename T, int CFG>
+inline int cgemm_occ_cfg() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_cgemm<T, CFG>), 256, 0) !=
+        hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+template <typename T>
+inline int cgemm_occupancy(int cfg) {
+    switch (cfg) {
+        case 1: return cgemm_occ_cfg<T, 1>();
+        case 2: return cgemm_occ_cfg<T, 2>();
+        case 3: return cgemm_occ_cfg<T, 3>();
+        default: return cgemm_occ_cfg<T, 0>();
+    }
+}
+static_assert(kGemmCfgs <= GemmPlan::kMaxCfgs, "GemmPlan holds one occupancy per configuration");
+
+// Grid cap that leaves one block of the configuration free on every CU: a
+// trailing update launched this way never takes all of a CU's LDS, so the
+// panel kernels running beside it on the other stream find room at once.
+inline long gemm_headroom_cap(const GemmPlan& p, int cfg) {
+    if (p.cap > 0) return p.cap;
+    const int per_cu = p.occ[cfg] > 1 ? p.occ[cfg] - 1 : 1;
+    return (long)p.cus * per_cu;
 }
 
 // Blocks of configuration cfg for an M x N product.
@@ -279,6 +361,7 @@ inline long gemm_blocks(int cfg, int M, int N) {
     switch (cfg) {
         case 1: bm = GemmCfg<T, 1>::BM; bn = GemmCfg<T, 1>::BN; break;
         case 2: bm = GemmCfg<T, 2>::BM; bn = GemmCfg<T, 2>::BN; break;
+        case 3: bm = GemmCfg<T, 3>::BM; bn = GemmCfg<T, 3>::BN; break;
         default: bm = GemmCfg<T, 0>::BM; bn = GemmCfg<T, 0>::BN; break;
     }
     return (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
@@ -292,17 +375,27 @@ inline int gemm_choose(const bem_ctx* c, int M, int N) {
     return gemm_blocks<T>(p.large, M, N) >= p.fill ? p.large : p.skinny;
 }
 
+// Launch flags of cgemm_ctx: GEMM_HEADROOM caps the grid as
+// gemm_headroom_cap does (an update running beside the panel stream);
+// GEMM_PANEL marks a product of the panel stream's critical path: the plan's
+// panel configuration at raised issue priority.
+constexpr int GEMM_HEADROOM = 1, GEMM_PANEL = 2;
+
 // GEMM with optional per-launch timing, on stream s (default: the context
 // stream), tile configuration from the context's plan.  Counted work: 3M
 // form 6 M N K real flops (FP32), 4M form 8 M N K (FP64); bytes = operands
 // read once plus C read and written.
 template <typename T>
 inline void cgemm_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br, const T* Bi,
-                      long ldb, T* Cr, T* Ci, long ldc, int mode, hipStream_t s = nullptr) {
+                      long ldb, T* Cr, T* Ci, long ldc, int mode, hipStream_t s = nullptr, int flags = 0) {
     if (M <= 0 || N <= 0 || K <= 0) return;
     if (!s) s = c->stream;
     const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, s);
-    cgemm_any<T>(gemm_choose<T>(c, M, N), s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+    const GemmPlan& plan = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
+    const int cfg = (flags & GEMM_PANEL) ? plan.panel : gemm_choose<T>(c, M, N);
+    const long cap = (flags & GEMM_HEADROOM) ? gemm_headroom_cap(plan, cfg) : 0;
+    if (flags & GEMM_PANEL) mode |= GEMM_MODE_URGENT;
+    cgemm_any<T>(cfg, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, cap);
     const double mnk = (double)M * N * K;
     const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
     kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);

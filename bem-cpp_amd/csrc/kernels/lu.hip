@@ -19,7 +19,8 @@
 //     3. the 32 x 32 diagonal block is factored without further pivoting,
 //        L21 = A21 U11^{-1} row by row, U12 of the rest of the outer panel
 //     4. rank-32 update of the rest of the outer panel on the matrix cores
-//   then L11^{-1} of the 256 x 256 diagonal block (trsm.h), U12 = L11^{-1} A12
+//   then L11^{-1} of the 256 x 256 diagonal block (by doubling: eight 32 x 32
+//   inverses in LDS, then three levels of batched GEMMs), U12 = L11^{-1} A12
 //   and the rank-256 trailing update A22 -= L21 U12 as matrix-core GEMMs
 //   (gemm.h; 3M form in FP32).  The inverses of all L and U diagonal blocks
 //   are kept for the GEMM-based multi-RHS solves.
@@ -249,6 +250,7 @@ __global__ __launch_bounds__(Tslu<T>::CH) void k_tslu(const T* __restrict__ Ar, 
                                                       int* __restrict__ ipiv, int* __restrict__ pl,
                                                       T* __restrict__ d11r, T* __restrict__ d11i,
                                                       int* __restrict__ info) {
+    __builtin_amdgcn_s_setprio(3);  // latency-bound: win issue slots over co-resident GEMM waves
     constexpr int CH = Tslu<T>::CH, G = CH / PB;
     __shared__ ElimLds<T, CH / 64> L;
     const int t = threadIdx.x;
@@ -331,14 +333,18 @@ __global__ __launch_bounds__(Tslu<T>::CH) void k_tslu(const T* __restrict__ Ar, 
 // Applies nl row-pair lists in order (list l at pl + l * PLS, each the net
 // effect of one panel step's swaps) to the columns [ca, cb) and [cc, cd): per
 // list, all sources are read into LDS before any destination is written.
-// 64 columns per block, 4 row groups.
+// 32 columns per block (128-byte row segments in FP32), 8 row groups; the
+// 16 / 32 KB of LDS fit beside a capped trailing-update GEMM.
 constexpr int PLS = 256;  // ints per row-pair list
+constexpr int LASWP_COLS = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict__ Ai, long ld, int ca, int cb, int cc,
                                                int cd, const int* __restrict__ pl, int nl) {
-    __shared__ T br[2 * PB][64], bi[2 * PB][64];
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int v = blockIdx.x * 64 + tx;  // virtual column over both ranges
+    __builtin_amdgcn_s_setprio(3);
+    constexpr int RG = 256 / LASWP_COLS;  // row groups
+    __shared__ T br[2 * PB][LASWP_COLS], bi[2 * PB][LASWP_COLS];
+    const int tx = threadIdx.x % LASWP_COLS, ty = threadIdx.x / LASWP_COLS;
+    const int v = blockIdx.x * LASWP_COLS + tx;  // virtual column over both ranges
     const int w1 = cb - ca;
     const int c = v < w1 ? ca + v : cc + (v - w1);
     const bool ok = v < w1 + (cd - cc);
@@ -346,14 +352,14 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict
         const int* q = pl + l * PLS;
         const int np = q[0];
         if (np == 0) continue;
-        for (int k = ty; k < np; k += 4)
+        for (int k = ty; k < np; k += RG)
             if (ok) {
                 const long o = (long)q[2 + 2 * k] * ld + c;
                 br[k][tx] = Ar[o];
                 bi[k][tx] = Ai[o];
             }
         __syncthreads();
-        for (int k = ty; k < np; k += 4)
+        for (int k = ty; k < np; k += RG)
             if (ok) {
                 const long o = (long)q[1 + 2 * k] * ld + c;
                 Ar[o] = br[k][tx];
@@ -376,6 +382,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict__ Ai, long ld, int j0, int nrows,
                                                int nright, const T* __restrict__ d11r,
                                                const T* __restrict__ d11i) {
+    __builtin_amdgcn_s_setprio(3);
     using V = typename Vec16<T>::type;
     constexpr int E = 16 / sizeof(T);
     __shared__ __align__(16) T ur[PB][PB], ui[PB][PB];  // U11 (rows)
@@ -484,6 +491,7 @@ struct LuWork {
     int* pl;         // row-pair lists of the 8 steps of the current outer panel (PLS ints each)
     T *d11r, *d11i;  // factored 32 x 32 diagonal block
     T *ubr, *ubi;    // 256 x n U12 buffer
+    T *tr, *ti;      // 128 x 128 product of the L11^{-1} doubling
 };
 
 template <typename T>
@@ -492,8 +500,37 @@ void launch_laswp(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, long ld, int ca, int 
     const int ncols = (cb - ca) + (cd - cc);
     if (ncols <= 0) return;
     const int kt = kt_mark(c, KS_LASWP, s);
-    hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 63) / 64), dim3(256), 0, s, Ar, Ai, ld, ca, cb, cc, cd, pl, nl);
+    hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + LASWP_COLS - 1) / LASWP_COLS), dim3(256), 0, s, Ar, Ai, ld, ca, cb,
+                       cc, cd, pl, nl);
     kt_close(c, kt, 0.0, 2.0 * 2.0 * sizeof(T) * (double)ncols * 2 * PB * nl);
+}
+
+// L11^{-1} of the NB x NB unit lower triangle at F (ld ldf) into I (ld NB) by
+// doubling.  The inverses of the eight 32 x 32 diagonal blocks come from one
+// LDS substitution launch; then for s = 32, 64, 128 every pair of s x s
+// diagonal blocks is joined,
+//     [X11 0; X21 X22] with X21 = -X22 (L21 X11),
+// as two batched GEMMs (all pairs of a level in one launch each, on the tile
+// with the smallest LDS footprint, since they run beside the trailing update).  The blocks
+// above the diagonal are not written: the caller keeps them zero.  Seven
+// launches with short dependency chains replace one 256-step substitution.
+template <typename T>
+void invert_lower_doubling(hipStream_t s, const T* Fr, const T* Fi, long ldf, T* Ir, T* Ii, const LuWork<T>& w) {
+    constexpr int CW = TrsmCW<T>::value;
+    constexpr int D = 32;
+    hipLaunchKernelGGL((k_trsm_slab<T, CW, true, D>), dim3(D / CW, NB / D), dim3(256), 0, s, D, Fr, Fi, ldf,
+                       (long)D * (ldf + 1), nullptr, nullptr, 0L, 0L, Ir, Ii, (long)NB, (long)D * (NB + 1), D);
+    for (int h = D; h < NB; h *= 2) {
+        const int pairs = NB / (2 * h);
+        const long sf = 2L * h * (ldf + 1), si = 2L * h * (NB + 1), st = (long)h * h;
+        const long f21 = (long)h * ldf, i21 = (long)h * NB, i22 = (long)h * (NB + 1);
+        // T = L21 X11, then X21 = -X22 T
+        cgemm_any<T>(3, s, h, h, h, Fr + f21, Fi + f21, ldf, Ir, Ii, NB, w.tr, w.ti, h, 1 | GEMM_MODE_URGENT, pairs,
+                     sf, si, st);
+        cgemm_any<T>(3, s, h, h, h, Ir + i22, Ii + i22, NB, w.tr, w.ti, h, Ir + i21, Ii + i21, NB, 3 | GEMM_MODE_URGENT,
+                     pairs, si, st,
+                     si);
+    }
 }
 
 // Factors the outer panel of columns k0 .. k0+NB-1 (rows k0 .. n-1) on stream
@@ -522,7 +559,7 @@ void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<
             const long o21 = (long)(j0 + PB) * ld + j0, o12 = (long)j0 * ld + j0 + PB;
             const long o22 = (long)(j0 + PB) * ld + j0 + PB;
             cgemm_ctx<T>(c, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22,
-                         ld, 0, s);
+                         ld, 0, s, GEMM_PANEL);
         }
     }
 }
@@ -531,7 +568,7 @@ void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<
 // of outer panel k0, on stream us; U12 goes through the work buffer.
 template <typename T>
 void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, const T* Lii, int k0, int c0, int nc,
-                    const LuWork<T>& w) {
+                    const LuWork<T>& w, bool headroom) {
     const int n = c->npad;
     const long ld = n;
     const int nr = n - (k0 + NB);
@@ -540,7 +577,8 @@ void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, cons
     T* ubr = w.ubr + (c0 - (k0 + NB));
     T* ubi = w.ubi + (c0 - (k0 + NB));
     cgemm_ctx<T>(c, NB, nc, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1, us);
-    cgemm_ctx<T>(c, nr, nc, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0, us);
+    cgemm_ctx<T>(c, nr, nc, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0, us,
+                 headroom ? GEMM_HEADROOM : 0);
     const int kt = kt_mark(c, KS_OTHER, us);
     hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)(((long)NB * nc + 255) / 256)), dim3(256), 0, us, NB, nc, ubr,
                        ubi, ld, Ar + o12, Ai + o12, ld);
@@ -551,11 +589,15 @@ void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, cons
 // look-ahead of depth one: outer panel K+1 is factored on the panel stream
 // while the trailing update of panel K runs on the update stream.  Panel K+1
 // only touches its own columns; its swaps reach the other columns on the
-// update stream once the update of panel K has finished.  Stream layout
-// (c->lu_streams): 0 = everything on the context stream (no overlap);
-// 1 = panel stream and context stream over all CUs; 2 = the panel stream
-// confined to a reserved set of CUs that the update stream does not use, so
-// the latency-bound panel kernels never share a CU with the trailing GEMM.
+// update stream.  After panel K, the critical path to panel K+1 is: swaps in
+// the columns of panel K+1 (update stream) beside L11^{-1} (panel stream),
+// then U12 and the update of those 256 columns; the swaps and the update of
+// all other columns follow on the update stream while panel K+1 is factored.
+// Stream layout (c->lu_streams): 0 = everything on the context stream (no
+// overlap); 1 = panel stream and context stream over all CUs; 2 = the panel
+// stream confined to a reserved set of CUs that the update stream does not
+// use, so the latency-bound panel kernels never share a CU with the trailing
+// GEMM.
 template <typename T>
 void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWork<T>& w) {
     const int n = c->npad, nb = n / NB;
@@ -566,30 +608,37 @@ void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWo
         us = c->stream_upd;
         ps = c->stream_pnl;
     }
-    hipEvent_t e_panel = c->lu_ev[0], e_next = c->lu_ev[1], e_start = c->lu_ev[2];
+    hipEvent_t e_panel = c->lu_ev[0], e_next = c->lu_ev[1], e_start = c->lu_ev[2], e_linv = c->lu_ev[3];
     (void)hipEventRecord(e_start, c->stream);  // the matrix and work buffers are ready
     (void)hipStreamWaitEvent(ps, e_start, 0);
     if (us != c->stream) (void)hipStreamWaitEvent(us, e_start, 0);
     factor_panel<T>(c, ps, Ar, Ai, 0, w);
     for (int K = 0; K < nb; K++) {
         const int k0 = K * NB;
-        (void)hipEventRecord(e_panel, ps);
-        (void)hipStreamWaitEvent(us, e_panel, 0);
-        // swaps of panel K in every other column, then L11^{-1}
-        launch_laswp<T>(c, us, Ar, Ai, ld, 0, k0, k0 + NB, n, w.pl, NB / PB);
+        const int nr = n - (k0 + NB);
         const long o11 = (long)k0 * ld + k0;
         const long io = (long)K * NB * NB;
-        int kt = kt_mark(c, KS_TRSM, us);
-        invert_diag_blocks<T>(us, NB, 1, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, true);
+        (void)hipEventRecord(e_panel, ps);
+        int kt = kt_mark(c, KS_TRSM, ps);
+        invert_lower_doubling<T>(ps, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, w);
         kt_close(c, kt, 8.0 * NB * NB * NB / 6.0, 4.0 * sizeof(T) * (double)NB * NB);
-        const int nr = n - (k0 + NB);
-        if (nr <= 0) break;
+        (void)hipEventRecord(e_linv, ps);
+        (void)hipStreamWaitEvent(us, e_panel, 0);
+        if (nr <= 0) {
+            launch_laswp<T>(c, us, Ar, Ai, ld, 0, k0, 0, 0, w.pl, NB / PB);
+            (void)hipStreamWaitEvent(us, e_linv, 0);
+            break;
+        }
         // columns of panel K+1 first, then hand panel K+1 to the panel stream
-        update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + NB, NB, w);
+        launch_laswp<T>(c, us, Ar, Ai, ld, k0 + NB, k0 + 2 * NB, 0, 0, w.pl, NB / PB);
+        (void)hipStreamWaitEvent(us, e_linv, 0);
+        update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + NB, NB, w, false);
         (void)hipEventRecord(e_next, us);
         (void)hipStreamWaitEvent(ps, e_next, 0);
         factor_panel<T>(c, ps, Ar, Ai, k0 + NB, w);
-        if (nr > NB) update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + 2 * NB, nr - NB, w);
+        launch_laswp<T>(c, us, Ar, Ai, ld, 0, k0, k0 + 2 * NB, n, w.pl, NB / PB);
+        if (nr > NB)
+            update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + 2 * NB, nr - NB, w, ps != us && c->gemm_plan[sizeof(T) == 4 ? 0 : 1].cap > 0);
     }
     // U^{-1} of all diagonal blocks in one launch
     const int kt = kt_mark(c, KS_TRSM, us);
@@ -629,7 +678,8 @@ int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     const size_t ncnt = 2 * (size_t)nsets0 + 64;
     const size_t nint = 64 + ncnt + (NB / PB) * PLS + 2 * (size_t)nsets0 * PB;
     const size_t bytes = nint * sizeof(int) + 256 + 4 * (size_t)nsets0 * PB * PB * sizeof(T) +
-                         2 * (size_t)PB * PB * sizeof(T) + 2 * (size_t)NB * n * sizeof(T) + 256;
+                         2 * (size_t)PB * PB * sizeof(T) + 2 * (size_t)NB * n * sizeof(T) +
+                         2 * (size_t)(NB / 2) * (NB / 2) * sizeof(T) + 256;
     void* scr = nullptr;
     if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
     LuWork<T> w;
@@ -645,7 +695,11 @@ int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     w.d11i = w.d11r + PB * PB;
     w.ubr = w.d11i + PB * PB;
     w.ubi = w.ubr + (size_t)NB * n;
+    w.tr = w.ubi + (size_t)NB * n;
+    w.ti = w.tr + (size_t)(NB / 2) * (NB / 2);
     BEM_HIP(c, hipMemsetAsync(w.info, 0, (64 + ncnt) * sizeof(int), c->stream));
+    // the doubling writes only the diagonal and lower blocks of each L11^{-1}
+    BEM_HIP(c, hipMemsetAsync(Lir, 0, 2 * (size_t)n * NB * sizeof(T), c->stream));
     factor<T>(c, Ar, Ai, Lir, Lii, Uir, Uii, w);
     return 0;
 }
@@ -686,23 +740,94 @@ int lu_setup_streams(bem_ctx* c) {
     return 0;
 }
 
+namespace {
+
+// Systems from this size on choose the trailing-update tile by whole trial
+// factorisations: how well the update GEMM overlaps the panel stream decides
+// the factorisation time, and a GEMM timed alone does not show that.
+constexpr int kTrialMinN = 8192;
+
+// The working copy of precision T from the FP64 matrix Z.
+template <typename T>
+int load_working_copy(bem_ctx* c, T* Ar, T* Ai) {
+    const size_t nn = (size_t)c->npad * c->npad;
+    if constexpr (sizeof(T) == 8) {
+        BEM_HIP(c, hipMemcpyAsync(Ar, c->Zr, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        BEM_HIP(c, hipMemcpyAsync(Ai, c->Zi, nn * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        const size_t blocks = (nn + 255) / 256;
+        hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c->stream, c->Zr, Ar, nn);
+        hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, c->stream, c->Zi, Ai, nn);
+    }
+    return 0;
+}
+
+// One timed factorisation of the context's matrix per tile configuration;
+// the fastest becomes the plan's trailing-update tile.  Every configuration
+// gives the same factors (same accumulation order), so the trials change
+// nothing but the choice.  Sets *ran when the working copy was overwritten.
+template <typename T>
+int factor_trials(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes, bool* ran) {
+    GemmPlan& plan = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
+    *ran = false;
+    if (plan.trialed) return 0;
+    if (c->npad < kTrialMinN) {
+        plan.trialed = true;
+        return 0;
+    }
+    if (!c->has_Z64) return 0;  // no source to restore the working copy from
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return bem_fail(c, BEM_ERR_HIP, "LU trials: events");
+    }
+    *ran = true;
+    int best = plan.large, rc = 0;
+    float best_ms = 0.0f;
+    for (int cfg = 0; cfg < kGemmCfgs && !rc; cfg++) {
+        plan.large = cfg;
+        if ((rc = load_working_copy<T>(c, Ar, Ai))) break;
+        (void)hipEventRecord(e0, c->stream);
+        if ((rc = factor_prec<T>(c, Ar, Ai, inv, inv_bytes))) break;
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = bem_fail(c, BEM_ERR_HIP, "LU trials: factorisation");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        plan.trial_ms[cfg] = ms;
+        if (cfg == 0 || ms < best_ms) {
+            best = cfg;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    plan.large = best;
+    if (rc) return rc;
+    plan.trialed = true;
+    gemm_cache_store(c, sizeof(T) == 4);
+    return 0;
+}
+
+}  // namespace
+
 int lu_factor_launch(bem_ctx* c, int precision) {
     const int n = c->npad;
-    const size_t nn = (size_t)n * n;
     hipStream_t s = c->stream;
     int rc = gemm_ensure_tuned(c, precision != BEM_F64);
     if (rc) return rc;
     if ((rc = lu_setup_streams(c))) return rc;
+    bool trials = false;
     if (precision == BEM_F64) {
-        BEM_HIP(c, hipMemcpyAsync(c->Vr, c->Zr, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
-        BEM_HIP(c, hipMemcpyAsync(c->Vi, c->Zi, nn * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if ((rc = factor_trials<double>(c, c->Vr, c->Vi, c->inv64, c->inv64_bytes, &trials))) return rc;
+        if ((rc = load_working_copy<double>(c, c->Vr, c->Vi))) return rc;
         rc = factor_prec<double>(c, c->Vr, c->Vi, c->inv64, c->inv64_bytes);
     } else {
-        if (!c->has_Z32) {
-            const size_t blocks = (nn + 255) / 256;
-            hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, s, c->Zr, c->Wr, nn);
-            hipLaunchKernelGGL(k_convert<float>, dim3(blocks), dim3(256), 0, s, c->Zi, c->Wi, nn);
-        }
+        if ((rc = factor_trials<float>(c, c->Wr, c->Wi, c->inv32, c->inv32_bytes, &trials))) return rc;
+        if (!c->has_Z32 || trials)
+            if ((rc = load_working_copy<float>(c, c->Wr, c->Wi))) return rc;
         rc = factor_prec<float>(c, c->Wr, c->Wi, c->inv32, c->inv32_bytes);
     }
     if (rc) return rc;

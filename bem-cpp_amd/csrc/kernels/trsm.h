@@ -20,12 +20,15 @@ __device__ __forceinline__ void cdiv(T ar, T ai, T br, T bi, T& qr, T& qi) {
     qi = (ai * br - ar * bi) / d;
 }
 
-template <typename T, int CW, bool LOWER>
+// RMAX: largest R of the launch (sizes the LDS slab; 32 for the 32 x 32 blocks
+// of the doubling inverse, so that launch fits beside a running GEMM).
+template <typename T, int CW, bool LOWER, int RMAX = 256>
 __global__ __launch_bounds__(256) void k_trsm_slab(int R, const T* __restrict__ Tr, const T* __restrict__ Ti,
                                                    long ldt, long bst, const T* __restrict__ Br,
                                                    const T* __restrict__ Bi, long ldb, long bsb, T* __restrict__ Xr,
                                                    T* __restrict__ Xi, long ldx, long bsx, int ncols) {
-    __shared__ T sr[256][33], si[256][33];
+    __builtin_amdgcn_s_setprio(3);  // on the factorisation's critical path beside the trailing GEMM
+    __shared__ T sr[RMAX][33], si[RMAX][33];
     __shared__ T xr_s[2][CW], xi_s[2][CW];
     const int t = threadIdx.x, c0 = blockIdx.x * CW;
     Tr += blockIdx.y * bst;

@@ -35,8 +35,9 @@ struct Shape {
 };
 
 // Best configuration and its rate (TFLOP/s) for one shape; rates[cfg] filled.
+// With a plan, every configuration runs under its headroom grid cap.
 template <typename T>
-int time_configs(hipStream_t s, const Shape& sh, T* buf, double* rates) {
+int time_configs(hipStream_t s, const Shape& sh, T* buf, double* rates, const GemmPlan* capped) {
     T* Ar = buf;
     T* Ai = Ar + (size_t)sh.M * sh.K;
     T* Br = Ai + (size_t)sh.M * sh.K;
@@ -53,7 +54,8 @@ int time_configs(hipStream_t s, const Shape& sh, T* buf, double* rates) {
     int best = 0;
     for (int cfg = 0; cfg < kGemmCfgs; cfg++) {
         auto run = [&]() {
-            cgemm_any<T>(cfg, s, sh.M, sh.N, sh.K, Ar, Ai, sh.K, Br, Bi, sh.N, Cr, Ci, sh.N, 0);
+            cgemm_any<T>(cfg, s, sh.M, sh.N, sh.K, Ar, Ai, sh.K, Br, Bi, sh.N, Cr, Ci, sh.N, 0, 1, 0, 0, 0,
+                         capped ? gemm_headroom_cap(*capped, cfg) : 0);
         };
         run();  // warm-up (code load, caches)
         const int reps = 3;
@@ -97,17 +99,22 @@ int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* r
     if (hipMalloc(&buf, elems * sizeof(T)) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "autotune buffers");
     hipLaunchKernelGGL(k_fill_tune<T>, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, c->stream, buf,
                        (long)elems, 7u);
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    plan.cus = cus;
+    for (int i = 0; i < kGemmCfgs; i++) plan.occ[i] = cgemm_occupancy<T>(i);
     double rl[kGemmCfgs] = {0}, rs[kGemmCfgs] = {0};
-    const int bl = time_configs<T>(c->stream, large, buf, rl);
-    const int bs = time_configs<T>(c->stream, skinny, buf, rs);
+    const int bl = time_configs<T>(c->stream, large, buf, rl, nullptr);
+    const int bs = time_configs<T>(c->stream, skinny, buf, rs, nullptr);
     const hipError_t e = hipStreamSynchronize(c->stream);
     (void)hipFree(buf);
     if (e != hipSuccess || bl < 0 || bs < 0)
         return bem_fail(c, BEM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     plan.large = bl;
     plan.skinny = bs;
+    plan.cap = 0;
+    plan.trialed = false;  // the first factorisation of this size times whole trials
+    for (double& t : plan.trial_ms) t = 0.0;
     plan.fill = 2L * cus;  // fewer blocks of the large tile than two per CU: use the skinny tile
     plan.tuned = true;
     plan.npad = npad;
@@ -126,6 +133,7 @@ int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* r
 int gemm_autotune_launch(bem_ctx* c, int npad, int nrhs, double* out, int nout) {
     const int ldr = round_up(std::max(nrhs, 1), 16);
     double r[4][kGemmCfgs] = {};
+    c->gemm_plan[0].cap = c->gemm_plan[1].cap = 0;
     const bool want = out && nout > 0;
     if (int rc = tune_precision<float>(c, npad, ldr, want ? r[0] : nullptr, want ? r[1] : nullptr)) return rc;
     if (int rc = tune_precision<double>(c, npad, ldr, want ? r[2] : nullptr, want ? r[3] : nullptr)) return rc;
@@ -143,6 +151,31 @@ int gemm_autotune_launch(bem_ctx* c, int npad, int nrhs, double* out, int nout) 
     return 0;
 }
 
+void gemm_cache_store(bem_ctx* c, bool f32) {
+    const GemmPlan& plan = c->gemm_plan[f32 ? 0 : 1];
+    const int ldr = round_up(std::max(c->ldr > 0 ? c->ldr : 128, 1), 16);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache[std::make_tuple(c->device, f32 ? 4 : 8, plan.npad, ldr)] = plan;
+}
+
+// Fixes the plan of one precision (0 FP32, 1 FP64) instead of tuning it.
+int gemm_set_plan(bem_ctx* c, int prec, int large, int skinny, long cap) {
+    if (prec < 0 || prec > 1 || large < 0 || large >= kGemmCfgs || skinny < 0 || skinny >= kGemmCfgs || cap < 0)
+        return bem_fail(c, BEM_ERR_ARG, "gemm_plan: bad configuration");
+    if (c->npad <= 0) return bem_fail(c, BEM_ERR_STATE, "gemm_plan: no system size yet (set a mesh or a matrix)");
+    GemmPlan& plan = c->gemm_plan[prec];
+    (void)hipDeviceGetAttribute(&plan.cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    for (int i = 0; i < kGemmCfgs; i++) plan.occ[i] = prec == 0 ? cgemm_occupancy<float>(i) : cgemm_occupancy<double>(i);
+    plan.large = large;
+    plan.skinny = skinny;
+    plan.cap = cap;
+    plan.tuned = true;
+    plan.trialed = true;  // fixed: no factorisation trials
+    for (double& t : plan.trial_ms) t = 0.0;
+    plan.npad = c->npad;
+    return 0;
+}
+
 // Tunes the precision about to be factored once per (device, size) unless the
 // environment sets BEM_AUTOTUNE=0 (then the defaults of GemmPlan stay).
 int gemm_ensure_tuned(bem_ctx* c, bool f32) {
@@ -150,7 +183,11 @@ int gemm_ensure_tuned(bem_ctx* c, bool f32) {
     if (plan.tuned && plan.npad == c->npad) return 0;
     const char* env = std::getenv("BEM_AUTOTUNE");
     if (env && env[0] == '0') {
+        (void)hipDeviceGetAttribute(&plan.cus, hipDeviceAttributeMultiprocessorCount, c->device);
+        for (int i = 0; i < kGemmCfgs; i++)
+            plan.occ[i] = f32 ? cgemm_occupancy<float>(i) : cgemm_occupancy<double>(i);
         plan.tuned = true;
+        plan.trialed = true;
         plan.npad = c->npad;
         return 0;
     }

@@ -80,12 +80,29 @@ int bem_kernel_stats(bem_ctx* ctx, int mode, double* out, int n);
  * the current system size with nrhs right-hand sides and keeps the fastest
  * (no counterpart in the reference, which leaves tiling to cuSOLVER; the
  * factorisation also tunes itself on its first call for a size unless
- * BEM_AUTOTUNE=0).  out (nullable, nout doubles) receives the measured
- * TFLOP/s: FP32 trailing-update shape per configuration, FP32 solve shape,
- * FP64 trailing-update shape, FP64 solve shape (3 configurations each), then
- * the chosen (large, skinny) configuration per precision.  Returns the
- * number of values written, or < 0. */
+ * BEM_AUTOTUNE=0).  out (nullable, nout doubles) receives the measured TFLOP/s: FP32
+ * trailing-update shape per configuration, FP32 solve shape, FP64
+ * trailing-update shape, FP64 solve shape (4 configurations each), then the
+ * chosen (large, skinny) configuration per precision.  Returns the number of
+ * values written, or < 0. */
 int bem_autotune(bem_ctx* ctx, int nrhs, double* out, int nout);
+
+/* Fixes the GEMM plan of one precision (prec 0 FP32, 1 FP64) instead of the
+ * autotuner's: configuration `large` for products with enough tiles to fill
+ * the device, `skinny` for the others; cap > 0 runs the trailing updates that
+ * overlap the panel factorisation on a persistent grid of cap blocks (0: one
+ * block per tile).  No counterpart in the reference; for experiments and
+ * tests.  Requires a system size; returns 0 or < 0. */
+int bem_gemm_plan(bem_ctx* ctx, int prec, int large, int skinny, long cap);
+
+/* The GEMM plan of one precision (prec 0 FP32, 1 FP64) as doubles: large,
+ * skinny and panel configuration, grid cap, tuned (0/1), chosen by trial
+ * factorisations (0/1), then the trial factorisation time in ms of each
+ * configuration (0 where none ran).  Systems of 8192 unknowns and more pick
+ * the trailing-update configuration on their first factorisation per
+ * precision by factoring once with each (no counterpart in the reference).
+ * Returns the number of values written, or < 0. */
+int bem_gemm_plan_get(bem_ctx* ctx, int prec, double* out, int nout);
 
 /* Copies `bytes` bytes (rounded down to 16) device to device with one plain
  * vectorised kernel: a transfer of known size, the scale for the HBM

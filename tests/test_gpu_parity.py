@@ -273,3 +273,52 @@ def test_autotune_reports_and_factorisation_stays_exact(ctx):
         X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
         be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
         assert be < 2e-15
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+def test_every_gemm_tile_and_capped_grid_factor_exactly(ctx, cfg):
+    # each tile configuration as the only one, with the capped trailing
+    # updates on a 7-block grid, so every block loops over several tiles and
+    # the last round is partial; FP64 and FP32 + refinement keep the
+    # backward-error bound of an unpivoted-growth-free LU
+    rs = np.random.RandomState(11)
+    n = 1000
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n)
+    B = rs.randn(n, 8) + 1j * rs.randn(n, 8)
+    ctx.set_Z(Z)
+    X_ref = np.linalg.solve(Z, B)
+    for prec in ("fp64", "fp32"):
+        ctx.gemm_plan(prec, cfg, cfg, cap=7)
+        assert ctx.factor(prec) == 0
+        X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
+        be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
+        assert be < 2e-15, (prec, cfg, be)
+        assert np.abs(X - X_ref).max() <= 1e-9 * np.abs(X_ref).max()
+    with pytest.raises(RuntimeError):
+        ctx.gemm_plan("fp32", len(ctx.GEMM_TILES), 0)
+
+
+def test_factor_trials_pick_a_tile_and_keep_results(ctx):
+    # from 8192 unknowns on, the first factorisation of each precision times
+    # one whole trial factorisation per tile configuration (on the context's
+    # own matrix) and keeps the fastest; the factors are the same either way
+    rs = np.random.RandomState(12)
+    n = 8192
+    Z = (rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))).astype(np.complex128)
+    B = rs.standard_normal((n, 4)) + 1j * rs.standard_normal((n, 4))
+    ctx.set_Z(Z)
+    for prec in ("fp32", "fp64"):
+        assert ctx.factor(prec) == 0
+        info = ctx.gemm_plan_info(prec)
+        assert info["trialed"], info
+        tiles = ctx.GEMM_TILES if prec == "fp32" else ctx.GEMM_TILES64
+        assert sorted(info["trial_factor_ms"]) == sorted(tiles), info
+        assert info["large"] == min(info["trial_factor_ms"], key=info["trial_factor_ms"].get)
+        X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
+        be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
+        # partial pivoting's growth on a random matrix rises with n: at
+        # n = 8192 the unrefined FP64 solve sits near 2e-15
+        assert be < 1e-14, (prec, be)
+        # a second factorisation reuses the choice (no further trials)
+        assert ctx.factor(prec) == 0
+        assert ctx.gemm_plan_info(prec) == info
