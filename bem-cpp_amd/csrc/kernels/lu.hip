@@ -103,11 +103,13 @@ __device__ __forceinline__ unsigned mag_key(double m) {
 }
 
 // Exchange area of the register-resident elimination for NW waves
-// (double-buffered by step parity so each step needs one barrier).
+// (double-buffered by step parity so each step needs one barrier).  Each
+// wave's candidate row is one 256 / 512-byte record (real parts, then
+// imaginary parts), so all of its reads use one base address and 16-byte
+// loads with immediate offsets.
 template <typename T, int NW>
 struct ElimLds {
-    T pr[2][NW][PB];  // best row of each wave (columns >= the step)
-    T pi[2][NW][PB];
+    alignas(16) T row[2][NW][2 * PB];  // best row of each wave (columns > the step)
     T rp[2][NW][2];   // 1 / its pivot element
     unsigned bk[2][NW];  // its key
     int bp[2][NW];       // its thread
@@ -138,11 +140,12 @@ __device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)
         const unsigned wk = __ockl_wfred_max_u32(key);
         const int p = wave * 64 + __ffsll((long long)__ballot(key == wk)) - 1;  // ties: lowest thread
         if (t == p) {
+            T* r = L.row[b][wave];
 #pragma unroll
             for (int c = 0; c < PB; c++)
                 if (c > jc) {
-                    L.pr[b][wave][c] = xr[c];
-                    L.pi[b][wave][c] = xi[c];
+                    r[c] = xr[c];
+                    r[PB + c] = xi[c];
                 }
             // reciprocal of the pivot (the rows scale by it, as zgetf2 does)
             const T d = xr[jc] * xr[jc] + xi[jc] * xi[jc];
@@ -173,12 +176,29 @@ __device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)
         const T li = upd ? xr[jc] * qi + xi[jc] * qr : T(0);
         xr[jc] = upd ? lr : xr[jc];
         xi[jc] = upd ? li : xi[jc];
+        // x_c -= l u_c as four fused multiply-adds per element, the pivot row
+        // read in 16-byte pieces from the aligned piece holding column jc + 1
+        const T nlr = -lr, nli = -li;
+        const T* r = L.row[b][pw];
+        constexpr int E = 16 / sizeof(T);
 #pragma unroll
-        for (int c = 0; c < PB; c++)
-            if (c > jc) {
-                const T ur = L.pr[b][pw][c], ui = L.pi[b][pw][c];
-                xr[c] -= lr * ur - li * ui;
-                xi[c] -= lr * ui + li * ur;
+        for (int q = 0; q < PB / E; q++)
+            if (q * E + E - 1 > jc) {
+                T ur[E], ui[E];
+                const typename Vec16<T>::type a = *reinterpret_cast<const typename Vec16<T>::type*>(r + q * E);
+                const typename Vec16<T>::type d = *reinterpret_cast<const typename Vec16<T>::type*>(r + PB + q * E);
+                __builtin_memcpy(ur, &a, 16);
+                __builtin_memcpy(ui, &d, 16);
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int c = q * E + e;
+                    if (c > jc) {
+                        xr[c] = fmadd(nlr, ur[e], xr[c]);
+                        xr[c] = fmadd(li, ui[e], xr[c]);
+                        xi[c] = fmadd(nlr, ui[e], xi[c]);
+                        xi[c] = fmadd(nli, ur[e], xi[c]);
+                    }
+                }
             }
     }
     if (mine >= 0) {
@@ -406,10 +426,12 @@ __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict
         }
     }
     __syncthreads();
-    // x_c -= s * M[l][c] for c > l, row l of M streamed from LDS
+    // x_c -= s * M[l][c] for c > l (four fused multiply-adds per element),
+    // row l of M streamed from LDS
     auto eliminate = [&](T (&xr)[PB], T (&xi)[PB], const T (*mr)[PB], const T (*mi)[PB], int l, T sr, T si) {
         const V* vr = reinterpret_cast<const V*>(&mr[l][0]);
         const V* vi = reinterpret_cast<const V*>(&mi[l][0]);
+        const T nsr = -sr, nsi = -si;
 #pragma unroll
         for (int q = 0; q < PB / E; q++) {
             if (q * E + E - 1 <= l) continue;
@@ -421,8 +443,10 @@ __global__ __launch_bounds__(256) void k_panel(T* __restrict__ Ar, T* __restrict
             for (int e = 0; e < E; e++) {
                 const int c = q * E + e;
                 if (c > l) {
-                    xr[c] -= sr * a[e] - si * b[e];
-                    xi[c] -= sr * b[e] + si * a[e];
+                    xr[c] = fmadd(nsr, a[e], xr[c]);
+                    xr[c] = fmadd(si, b[e], xr[c]);
+                    xi[c] = fmadd(nsr, b[e], xi[c]);
+                    xi[c] = fmadd(nsi, a[e], xi[c]);
                 }
             }
             __builtin_amdgcn_sched_barrier(0);

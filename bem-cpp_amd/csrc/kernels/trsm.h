@@ -13,6 +13,9 @@
 
 #include <hip/hip_runtime.h>
 
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ void cdiv(T ar, T ai, T br, T bi, T& qr, T& qi) {
     const T d = br * br + bi * bi;
@@ -93,11 +96,14 @@ __global__ __launch_bounds__(256) void k_trsm_slab(int R, const T* __restrict__ 
             __syncthreads();
             if (LOWER ? (t > l && t < R) : (t < l)) {
                 const T a = sr[t][l - s], b = si[t][l - s];
+                const T na = -a, nb = -b;
 #pragma unroll
                 for (int c = 0; c < CW; c++) {
                     const T ur = xr_s[buf][c], ui = xi_s[buf][c];
-                    xr[c] -= a * ur - b * ui;
-                    xi[c] -= a * ui + b * ur;
+                    xr[c] = fmadd(na, ur, xr[c]);
+                    xr[c] = fmadd(b, ui, xr[c]);
+                    xi[c] = fmadd(na, ui, xi[c]);
+                    xi[c] = fmadd(nb, ur, xi[c]);
                 }
             }
         }
