@@ -33,6 +33,7 @@ int ctx_scratch(bem_ctx* c, size_t bytes, void** out) {
 void kt_resolve(bem_ctx* c) {
     KernelTimer& k = c->kt;
     if (k.pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream3);
     (void)hipStreamSynchronize(c->stream2);
     if (c->stream_upd) (void)hipStreamSynchronize(c->stream_upd);
     if (c->stream_pnl) (void)hipStreamSynchronize(c->stream_pnl);
@@ -190,14 +191,19 @@ bem_ctx* bem_create(int device) {
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev[0]) != hipSuccess || hipEventCreate(&c->ev[1]) != hipSuccess ||
         hipEventCreateWithFlags(&c->lu_ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->lu_ev[1], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->lu_ev[2], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->lu_ev[3], hipEventDisableTiming) != hipSuccess) {
-        for (hipEvent_t e : {c->ev[0], c->ev[1], c->lu_ev[0], c->lu_ev[1], c->lu_ev[2], c->lu_ev[3]})
+        hipEventCreateWithFlags(&c->lu_ev[3], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->lu_ev[4], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->lu_ev[5], hipEventDisableTiming) != hipSuccess) {
+        for (hipEvent_t e : {c->ev[0], c->ev[1]})
             if (e) (void)hipEventDestroy(e);
-        for (hipStream_t s : {c->stream, c->stream2})
+        for (hipEvent_t e : c->lu_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t s : {c->stream, c->stream2, c->stream3})
             if (s) (void)hipStreamDestroy(s);
         delete c;
         return nullptr;
@@ -208,6 +214,7 @@ bem_ctx* bem_create(int device) {
 void bem_destroy(bem_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream3);
     (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream);
     if (c->stream_upd) (void)hipStreamSynchronize(c->stream_upd);
@@ -217,12 +224,14 @@ void bem_destroy(bem_ctx* c) {
     free_rhs(c);
     dfree(c->H);
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->part) (void)hipFree(c->part);
     (void)hipEventDestroy(c->ev[0]);
     (void)hipEventDestroy(c->ev[1]);
     for (hipEvent_t e : c->lu_ev) (void)hipEventDestroy(e);
     if (c->stream_upd) (void)hipStreamDestroy(c->stream_upd);
     if (c->stream_pnl) (void)hipStreamDestroy(c->stream_pnl);
     (void)hipStreamDestroy(c->stream2);
+    (void)hipStreamDestroy(c->stream3);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -89,6 +89,7 @@ struct GemmPlan {
     int npad = 0;                            // system size the plan was tuned for
     int cus = 256;                           // compute units of the device
     int occ[kMaxCfgs] = {1, 1, 1, 1};        // resident blocks per CU of each configuration
+    bool occ_known = false;                  // cus and occ queried from the device
     long cap = 0;                            // grid of capped launches; 0: from occ and cus
     bool trialed = false;                    // `large` chosen by whole trial factorisations
     double trial_ms[kMaxCfgs] = {0, 0, 0, 0};  // their times per configuration (0: not run)
@@ -102,13 +103,14 @@ struct bem_ctx {
     KernelTimer kt;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // panel stream of the look-ahead factorisation
+    hipStream_t stream3 = nullptr;  // updates inside an outer panel off its critical path
     // factorisation stream layout (see factor() in kernels/lu.hip): 0 one
     // stream, 1 panel + update streams on all CUs, 2 CU-partitioned
     int lu_streams = -1;            // -1: not chosen yet
     int lu_panel_cus = 0;           // CUs reserved for the panel stream (layout 2)
     hipStream_t stream_upd = nullptr, stream_pnl = nullptr;  // CU-masked streams of layout 2
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t lu_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // cross-stream ordering inside the factorisation
+    hipEvent_t lu_ev[6] = {};  // cross-stream ordering inside the factorisation
     std::string err;
 
     // mesh and basis (host copies)
@@ -160,6 +162,8 @@ struct bem_ctx {
     // scratch allocations owned by the kernels (grown on demand)
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    void* part = nullptr;  // partial products of split-K GEMMs
+    size_t part_bytes = 0;
 
     double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -202,6 +206,8 @@ int gemm_set_plan(bem_ctx* c, int prec, int large, int skinny, long cap);
 // Stores the context's plan of one precision in the per-process cache of
 // tuned plans (after the factorisation trials changed it).
 void gemm_cache_store(bem_ctx* c, bool f32);
+// Fills cus and occ of the plan of one precision from the device (no timing).
+void gemm_plan_occupancy(bem_ctx* c, bool f32);
 // Chooses the factorisation stream layout (environment BEM_LU_STREAMS,
 // BEM_LU_PANEL_CUS) and creates the CU-masked streams it needs.
 int lu_setup_streams(bem_ctx* c);

@@ -400,3 +400,86 @@ inline void cgemm_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai,
     const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
     kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);
 }
+
+// Sum of S split-K partial products (planes of M x N at P + 2 s M N, imaginary
+// plane M N further) into C: C -= sum (mode 0) or C = sum (mode 1), summed in
+// split order so results do not depend on scheduling.
+template <typename T>
+__global__ __launch_bounds__(256) void k_splitk_sum(long count, int N, int S, const T* __restrict__ P,
+                                                    T* __restrict__ Cr, T* __restrict__ Ci, long ldc, int mode) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    T vr = T(0), vi = T(0);
+    for (int s = 0; s < S; s++) {
+        vr += P[2 * s * count + idx];
+        vi += P[(2 * s + 1) * count + idx];
+    }
+    const long o = (idx / N) * ldc + idx % N;
+    if (mode == 0) {
+        Cr[o] -= vr;
+        Ci[o] -= vi;
+    } else {
+        Cr[o] = vr;
+        Ci[o] = vi;
+    }
+}
+
+// Splits for a product whose output tiles leave the device's block slots
+// unevenly used (few tiles, long K): the smallest S in {1, 2, 4, 8, 16} whose
+// last round of blocks is at least 90 % full (else the fullest), with every
+// K slice a multiple of 16 and at least 512 long.
+template <typename T>
+inline int gemm_splits(bem_ctx* c, int cfg, int M, int N, int K) {
+    const GemmPlan& p = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
+    const long tiles = gemm_blocks<T>(cfg, M, N);
+    const long slots = (long)p.cus * (p.occ[cfg] > 0 ? p.occ[cfg] : 1);
+    int best = 1;
+    double best_eff = 0.0;
+    for (int S = 1; S <= 16; S *= 2) {
+        if (S > 1 && (K % (16 * S) != 0 || K / S < 512)) break;
+        const long blocks = tiles * S;
+        const double eff = (double)blocks / (double)(slots * ((blocks + slots - 1) / slots));
+        if (eff > best_eff + 1e-9) {
+            best = S;
+            best_eff = eff;
+        }
+        if (eff >= 0.9) break;
+    }
+    return best;
+}
+
+// C = C - A B (mode 0) or C = A B (mode 1) with K split over S batched
+// products into the context's partial buffer, then summed (k_splitk_sum).
+// Falls back to one product when splitting does not pay.
+template <typename T>
+inline int cgemm_splitk_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
+                            const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
+    gemm_plan_occupancy(c, sizeof(T) == 4);
+    const int cfg = gemm_choose<T>(c, M, N);
+    const int S = gemm_splits<T>(c, cfg, M, N, K);
+    if (S == 1) {
+        cgemm_ctx<T>(c, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+        return 0;
+    }
+    const long count = (long)M * N;
+    const size_t bytes = 2 * (size_t)S * count * sizeof(T);
+    if (c->part_bytes < bytes) {
+        if (c->part) (void)hipFree(c->part);
+        c->part = nullptr;
+        c->part_bytes = 0;
+        if (hipMalloc(&c->part, bytes) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "split-K partial products");
+        c->part_bytes = bytes;
+    }
+    T* P = static_cast<T*>(c->part);
+    const int Ks = K / S;
+    const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, c->stream);
+    cgemm_any<T>(cfg, c->stream, M, N, Ks, Ar, Ai, lda, Br, Bi, ldb, P, P + count, N, 1, S, (long)Ks, (long)Ks * ldb,
+                 2 * count);
+    hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, count, N, S,
+                       (const T*)P, Cr, Ci, ldc, mode);
+    const double mnk = (double)M * N * K;
+    const double b = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (2.0 * S + (mode == 1 ? 1.0 : 2.0)) * count);
+    kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, b);
+    return 0;
+}

@@ -91,6 +91,11 @@ __device__ __forceinline__ void store_row(T* __restrict__ p, const T (&x)[PB]) {
     }
 }
 
+// 1 / d for the pivot scaling: the hardware reciprocal in FP32 (1 ulp), an
+// IEEE division in FP64.
+__device__ __forceinline__ float recip(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ double recip(double d) { return 1.0 / d; }
+
 // Wave-wide maximum of an unsigned value (DPP reduction of the device library).
 extern "C" __device__ unsigned __ockl_wfred_max_u32(unsigned);
 
@@ -102,18 +107,19 @@ __device__ __forceinline__ unsigned mag_key(double m) {
     return (unsigned)((unsigned long long)__double_as_longlong(m) >> 32) + 1u;
 }
 
-// Exchange area of the register-resident elimination for NW waves
-// (double-buffered by step parity so each step needs one barrier).  Each
-// wave's candidate row is one 256 / 512-byte record (real parts, then
-// imaginary parts), so all of its reads use one base address and 16-byte
-// loads with immediate offsets.
+// Exchange area of the register-resident elimination for NW waves.  A step
+// takes two barriers: the waves' best keys first, then only the winning
+// row.  (Every wave writing its candidate row before one barrier costs NW
+// times the LDS writes, which serialise on the CU's LDS and cost more than
+// the second barrier.)  Writes of a step happen after the previous step's
+// last barrier and its reads before the next step's first, so one buffer
+// suffices.
 template <typename T, int NW>
 struct ElimLds {
-    alignas(16) T row[2][NW][2 * PB];  // best row of each wave (columns > the step)
-    T rp[2][NW][2];   // 1 / its pivot element
-    unsigned bk[2][NW];  // its key
-    int bp[2][NW];       // its thread
-    int oid[PB], osrc[PB];  // chosen rows in pivot order
+    alignas(16) T row[2 * PB];  // the pivot row (columns > the step): real parts, then imaginary
+    T rp[2];                    // 1 / the pivot element
+    unsigned long long kb[NW];  // best key of each wave (high word) and ~its thread (low word)
+    int oid[PB], osrc[PB];      // chosen rows in pivot order
     int last;
     int zmin;  // first zero pivot of the root (1-based)
 };
@@ -134,44 +140,43 @@ __device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)
     int mine = -1;
 #pragma unroll
     for (int jc = 0; jc < PB; jc++) {
-        const int b = jc & 1;
         const bool open = valid && mine < 0;
         const unsigned key = open ? mag_key(mag2(xr[jc], xi[jc])) : 0u;
         const unsigned wk = __ockl_wfred_max_u32(key);
         const int p = wave * 64 + __ffsll((long long)__ballot(key == wk)) - 1;  // ties: lowest thread
-        if (t == p) {
-            T* r = L.row[b][wave];
+        if (t == p) L.kb[wave] = ((unsigned long long)wk << 32) | (0xffffffffu - (unsigned)p);
+        __syncthreads();
+        // largest key, ties to the lowest thread: one maximum over the packed words
+        unsigned long long kbest = L.kb[0];
 #pragma unroll
-            for (int c = 0; c < PB; c++)
-                if (c > jc) {
-                    r[c] = xr[c];
-                    r[PB + c] = xi[c];
+        for (int w = 1; w < NW; w++) kbest = max(kbest, L.kb[w]);
+        const unsigned best = (unsigned)(kbest >> 32);
+        const int pp = (int)(0xffffffffu - (unsigned)kbest);
+        if (t == pp) {
+            mine = jc;
+            // 16-byte stores from the aligned piece holding column jc + 1
+            using V = typename Vec16<T>::type;
+            constexpr int E = 16 / sizeof(T);
+#pragma unroll
+            for (int q = 0; q < PB / E; q++)
+                if (q * E + E - 1 > jc) {
+                    V a, d;
+                    __builtin_memcpy(&a, &xr[q * E], 16);
+                    __builtin_memcpy(&d, &xi[q * E], 16);
+                    *reinterpret_cast<V*>(L.row + q * E) = a;
+                    *reinterpret_cast<V*>(L.row + PB + q * E) = d;
                 }
             // reciprocal of the pivot (the rows scale by it, as zgetf2 does)
             const T d = xr[jc] * xr[jc] + xi[jc] * xi[jc];
-            const T inv = d > T(0) ? T(1) / d : T(0);
-            L.rp[b][wave][0] = xr[jc] * inv;
-            L.rp[b][wave][1] = -xi[jc] * inv;
-            L.bk[b][wave] = wk;
-            L.bp[b][wave] = p;
+            const T inv = d > T(0) ? recip(d) : T(0);
+            L.rp[0] = xr[jc] * inv;
+            L.rp[1] = -xi[jc] * inv;
         }
         __syncthreads();
-        unsigned best = L.bk[b][0];
-        int pp = L.bp[b][0], pw = 0;
-#pragma unroll
-        for (int w = 1; w < NW; w++) {
-            const unsigned bw = L.bk[b][w];
-            if (bw > best) {
-                best = bw;
-                pp = L.bp[b][w];
-                pw = w;
-            }
-        }
         // branch-free update: rows that are not eliminated use multiplier 0
         // (keeps one copy of the row in registers across the step)
         const bool upd = open && best > 1u && t != pp;
-        if (t == pp) mine = jc;
-        const T qr = L.rp[b][pw][0], qi = L.rp[b][pw][1];
+        const T qr = L.rp[0], qi = L.rp[1];
         const T lr = upd ? xr[jc] * qr - xi[jc] * qi : T(0);
         const T li = upd ? xr[jc] * qi + xi[jc] * qr : T(0);
         xr[jc] = upd ? lr : xr[jc];
@@ -179,14 +184,13 @@ __device__ __forceinline__ int elim_rows(ElimLds<T, NW>& L, T (&xr)[PB], T (&xi)
         // x_c -= l u_c as four fused multiply-adds per element, the pivot row
         // read in 16-byte pieces from the aligned piece holding column jc + 1
         const T nlr = -lr, nli = -li;
-        const T* r = L.row[b][pw];
         constexpr int E = 16 / sizeof(T);
 #pragma unroll
         for (int q = 0; q < PB / E; q++)
             if (q * E + E - 1 > jc) {
                 T ur[E], ui[E];
-                const typename Vec16<T>::type a = *reinterpret_cast<const typename Vec16<T>::type*>(r + q * E);
-                const typename Vec16<T>::type d = *reinterpret_cast<const typename Vec16<T>::type*>(r + PB + q * E);
+                const typename Vec16<T>::type a = *reinterpret_cast<const typename Vec16<T>::type*>(L.row + q * E);
+                const typename Vec16<T>::type d = *reinterpret_cast<const typename Vec16<T>::type*>(L.row + PB + q * E);
                 __builtin_memcpy(ur, &a, 16);
                 __builtin_memcpy(ui, &d, 16);
 #pragma unroll
@@ -351,10 +355,12 @@ __global__ __launch_bounds__(Tslu<T>::CH) void k_tslu(const T* __restrict__ Ar, 
 }
 
 // Applies nl row-pair lists in order (list l at pl + l * PLS, each the net
-// effect of one panel step's swaps) to the columns [ca, cb) and [cc, cd): per
-// list, all sources are read into LDS before any destination is written.
-// 32 columns per block (128-byte row segments in FP32), 8 row groups; the
-// 16 / 32 KB of LDS fit beside a capped trailing-update GEMM.
+// effect of one panel step's swaps, at most 2 PB pairs) to the columns
+// [ca, cb) and [cc, cd).  A block owns 32 columns and 8 row groups; each
+// thread reads the sources of its pairs of a list into registers (all loads
+// in flight together), the block synchronises, and the thread writes them to
+// the destinations, so every source is read before any destination of the
+// list is written.  No LDS, so the kernel fits beside a running GEMM.
 constexpr int PLS = 256;  // ints per row-pair list
 constexpr int LASWP_COLS = 32;
 template <typename T>
@@ -362,7 +368,7 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict
                                                int cd, const int* __restrict__ pl, int nl) {
     __builtin_amdgcn_s_setprio(3);
     constexpr int RG = 256 / LASWP_COLS;  // row groups
-    __shared__ T br[2 * PB][LASWP_COLS], bi[2 * PB][LASWP_COLS];
+    constexpr int PER = 2 * PB / RG;      // pairs per thread and list
     const int tx = threadIdx.x % LASWP_COLS, ty = threadIdx.x / LASWP_COLS;
     const int v = blockIdx.x * LASWP_COLS + tx;  // virtual column over both ranges
     const int w1 = cb - ca;
@@ -372,19 +378,26 @@ __global__ __launch_bounds__(256) void k_laswp(T* __restrict__ Ar, T* __restrict
         const int* q = pl + l * PLS;
         const int np = q[0];
         if (np == 0) continue;
-        for (int k = ty; k < np; k += RG)
-            if (ok) {
+        T vr[PER], vi[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int k = ty + RG * u;
+            if (ok && k < np) {
                 const long o = (long)q[2 + 2 * k] * ld + c;
-                br[k][tx] = Ar[o];
-                bi[k][tx] = Ai[o];
+                vr[u] = Ar[o];
+                vi[u] = Ai[o];
             }
+        }
         __syncthreads();
-        for (int k = ty; k < np; k += RG)
-            if (ok) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int k = ty + RG * u;
+            if (ok && k < np) {
                 const long o = (long)q[1 + 2 * k] * ld + c;
-                Ar[o] = br[k][tx];
-                Ai[o] = bi[k][tx];
+                Ar[o] = vr[u];
+                Ai[o] = vi[u];
             }
+        }
         __syncthreads();
     }
 }
@@ -561,10 +574,16 @@ void invert_lower_doubling(hipStream_t s, const T* Fr, const T* Fi, long ldf, T*
 // s: 8 steps of tournament, swaps within the panel's columns, L21/U12 of the
 // step and the rank-32 update of the panel's remaining columns.  The swaps of
 // step j are kept as row-pair list j for the columns outside the panel.
+// Look-ahead inside the panel: the rank-32 update of the next 32 columns
+// stays on s (the next tournament needs them); the update of the columns
+// beyond runs on stream ss and is joined before the next step's swaps, so
+// every column still receives its updates in step order.
 template <typename T>
-void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<T>& w) {
+void factor_panel(bem_ctx* c, hipStream_t s, hipStream_t ss, T* Ar, T* Ai, int k0, const LuWork<T>& w) {
     const int n = c->npad;
     const long ld = n;
+    hipEvent_t e_step = c->lu_ev[4], e_rest = c->lu_ev[5];
+    bool rest_pending = false;
     for (int j0 = k0; j0 < k0 + NB; j0 += PB) {
         const int m = n - j0;
         const int nsets = (m + Tslu<T>::CH - 1) / Tslu<T>::CH;
@@ -573,6 +592,10 @@ void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<
         hipLaunchKernelGGL(k_tslu<T>, dim3(nsets), dim3(Tslu<T>::CH), 0, s, Ar, Ai, ld, j0, n, w.cid, w.cvr, w.cvi, w.cnt,
                            c->ipiv, pl, w.d11r, w.d11i, w.info);
         kt_close(c, kt, 0.0, 2.0 * sizeof(T) * (double)m * PB);
+        if (rest_pending) {  // the previous step's update of the far columns
+            (void)hipStreamWaitEvent(s, e_rest, 0);
+            rest_pending = false;
+        }
         launch_laswp<T>(c, s, Ar, Ai, ld, k0, k0 + NB, 0, 0, pl, 1);
         const int nright = k0 + NB - (j0 + PB);
         const int nblk = 1 + (m - PB + 255) / 256;
@@ -582,8 +605,16 @@ void factor_panel(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, int k0, const LuWork<
         if (nright > 0) {
             const long o21 = (long)(j0 + PB) * ld + j0, o12 = (long)j0 * ld + j0 + PB;
             const long o22 = (long)(j0 + PB) * ld + j0 + PB;
-            cgemm_ctx<T>(c, m - PB, nright, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22,
-                         ld, 0, s, GEMM_PANEL);
+            if (nright > PB) {
+                (void)hipEventRecord(e_step, s);  // L21 and U12 of this step are in place
+                (void)hipStreamWaitEvent(ss, e_step, 0);
+                cgemm_ctx<T>(c, m - PB, nright - PB, PB, Ar + o21, Ai + o21, ld, Ar + o12 + PB, Ai + o12 + PB, ld,
+                             Ar + o22 + PB, Ai + o22 + PB, ld, 0, ss, GEMM_PANEL);
+                (void)hipEventRecord(e_rest, ss);
+                rest_pending = true;
+            }
+            cgemm_ctx<T>(c, m - PB, PB, PB, Ar + o21, Ai + o21, ld, Ar + o12, Ai + o12, ld, Ar + o22, Ai + o22, ld, 0,
+                         s, GEMM_PANEL);
         }
     }
 }
@@ -626,8 +657,8 @@ template <typename T>
 void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWork<T>& w) {
     const int n = c->npad, nb = n / NB;
     const long ld = n;
-    hipStream_t us = c->stream, ps = c->stream2;
-    if (c->lu_streams == 0) ps = us;
+    hipStream_t us = c->stream, ps = c->stream2, ss = c->stream3;
+    if (c->lu_streams == 0) ps = ss = us;
     if (c->lu_streams == 2) {
         us = c->stream_upd;
         ps = c->stream_pnl;
@@ -636,7 +667,7 @@ void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWo
     (void)hipEventRecord(e_start, c->stream);  // the matrix and work buffers are ready
     (void)hipStreamWaitEvent(ps, e_start, 0);
     if (us != c->stream) (void)hipStreamWaitEvent(us, e_start, 0);
-    factor_panel<T>(c, ps, Ar, Ai, 0, w);
+    factor_panel<T>(c, ps, ss, Ar, Ai, 0, w);
     for (int K = 0; K < nb; K++) {
         const int k0 = K * NB;
         const int nr = n - (k0 + NB);
@@ -659,7 +690,7 @@ void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWo
         update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + NB, NB, w, false);
         (void)hipEventRecord(e_next, us);
         (void)hipStreamWaitEvent(ps, e_next, 0);
-        factor_panel<T>(c, ps, Ar, Ai, k0 + NB, w);
+        factor_panel<T>(c, ps, ss, Ar, Ai, k0 + NB, w);
         launch_laswp<T>(c, us, Ar, Ai, ld, 0, k0, k0 + 2 * NB, n, w.pl, NB / PB);
         if (nr > NB)
             update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + 2 * NB, nr - NB, w, ps != us && c->gemm_plan[sizeof(T) == 4 ? 0 : 1].cap > 0);

@@ -157,7 +157,8 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
     auto residual = [&]() -> int {  // R = B - Z X, returns max relative residual in *res
         BEM_HIP(c, hipMemcpyAsync(Rr, c->Br, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
         BEM_HIP(c, hipMemcpyAsync(Ri, c->Bi, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
-        cgemm_ctx<double>(c, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0);
+        if (int rc = cgemm_splitk_ctx<double>(c, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0))
+            return rc;
         hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, s, c->n, Rr, Ri, (long)ldr, nr2);
         BEM_HIP(c, hipMemcpyAsync(r2.data(), nr2, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
         BEM_HIP(c, hipStreamSynchronize(s));

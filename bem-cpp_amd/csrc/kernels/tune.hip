@@ -99,10 +99,7 @@ int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* r
     if (hipMalloc(&buf, elems * sizeof(T)) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "autotune buffers");
     hipLaunchKernelGGL(k_fill_tune<T>, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, c->stream, buf,
                        (long)elems, 7u);
-    int cus = 256;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
-    plan.cus = cus;
-    for (int i = 0; i < kGemmCfgs; i++) plan.occ[i] = cgemm_occupancy<T>(i);
+    gemm_plan_occupancy(c, sizeof(T) == 4);
     double rl[kGemmCfgs] = {0}, rs[kGemmCfgs] = {0};
     const int bl = time_configs<T>(c->stream, large, buf, rl, nullptr);
     const int bs = time_configs<T>(c->stream, skinny, buf, rs, nullptr);
@@ -115,7 +112,7 @@ int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* r
     plan.cap = 0;
     plan.trialed = false;  // the first factorisation of this size times whole trials
     for (double& t : plan.trial_ms) t = 0.0;
-    plan.fill = 2L * cus;  // fewer blocks of the large tile than two per CU: use the skinny tile
+    plan.fill = 2L * plan.cus;  // fewer blocks of the large tile than two per CU: use the skinny tile
     plan.tuned = true;
     plan.npad = npad;
     if (rates_large)
@@ -151,6 +148,14 @@ int gemm_autotune_launch(bem_ctx* c, int npad, int nrhs, double* out, int nout) 
     return 0;
 }
 
+void gemm_plan_occupancy(bem_ctx* c, bool f32) {
+    GemmPlan& plan = c->gemm_plan[f32 ? 0 : 1];
+    if (plan.occ_known) return;
+    (void)hipDeviceGetAttribute(&plan.cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    for (int i = 0; i < kGemmCfgs; i++) plan.occ[i] = f32 ? cgemm_occupancy<float>(i) : cgemm_occupancy<double>(i);
+    plan.occ_known = true;
+}
+
 void gemm_cache_store(bem_ctx* c, bool f32) {
     const GemmPlan& plan = c->gemm_plan[f32 ? 0 : 1];
     const int ldr = round_up(std::max(c->ldr > 0 ? c->ldr : 128, 1), 16);
@@ -164,8 +169,7 @@ int gemm_set_plan(bem_ctx* c, int prec, int large, int skinny, long cap) {
         return bem_fail(c, BEM_ERR_ARG, "gemm_plan: bad configuration");
     if (c->npad <= 0) return bem_fail(c, BEM_ERR_STATE, "gemm_plan: no system size yet (set a mesh or a matrix)");
     GemmPlan& plan = c->gemm_plan[prec];
-    (void)hipDeviceGetAttribute(&plan.cus, hipDeviceAttributeMultiprocessorCount, c->device);
-    for (int i = 0; i < kGemmCfgs; i++) plan.occ[i] = prec == 0 ? cgemm_occupancy<float>(i) : cgemm_occupancy<double>(i);
+    gemm_plan_occupancy(c, prec == 0);
     plan.large = large;
     plan.skinny = skinny;
     plan.cap = cap;
@@ -183,9 +187,7 @@ int gemm_ensure_tuned(bem_ctx* c, bool f32) {
     if (plan.tuned && plan.npad == c->npad) return 0;
     const char* env = std::getenv("BEM_AUTOTUNE");
     if (env && env[0] == '0') {
-        (void)hipDeviceGetAttribute(&plan.cus, hipDeviceAttributeMultiprocessorCount, c->device);
-        for (int i = 0; i < kGemmCfgs; i++)
-            plan.occ[i] = f32 ? cgemm_occupancy<float>(i) : cgemm_occupancy<double>(i);
+        gemm_plan_occupancy(c, f32);
         plan.tuned = true;
         plan.trialed = true;
         plan.npad = c->npad;
