@@ -315,13 +315,14 @@ class Context:
         self._check(self._L.bem_device_copy(self._h, float(nbytes)), "device_copy")
 
     KERNEL_CLASSES = ("gemm_fp32_3m", "gemm_fp64", "asm_far", "asm_near", "asm_gather", "tslu", "laswp", "panel",
-                      "trsm", "farfield", "rhs", "other")
+                      "trsm", "farfield", "rhs", "other", "gemm_fp32_update", "gemm_fp64_update")
 
     def kernel_stats(self, mode=None):
-        """mode 'on' / 'off' reset the counters; None returns
+        """mode 'on' (every class), 'update' (the LU trailing-update GEMMs
+        only) and 'off' reset the counters; None returns
         {class: (launches, ms, flops, bytes)}."""
-        if mode in ("on", "off"):
-            self._L.bem_kernel_stats(self._h, 1 if mode == "on" else 0, None, 0)
+        if mode in ("on", "off", "update"):
+            self._L.bem_kernel_stats(self._h, {"on": 1, "off": 0, "update": 3}[mode], None, 0)
             return None
         out = np.zeros(4 * len(self.KERNEL_CLASSES))
         n = self._L.bem_kernel_stats(self._h, 2, _p(out), len(self.KERNEL_CLASSES))

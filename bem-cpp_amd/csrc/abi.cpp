@@ -249,8 +249,9 @@ int bem_kernel_stats(bem_ctx* c, int mode, double* out, int n) {
     if (!c) return BEM_ERR_ARG;
     KernelTimer& k = c->kt;
     kt_resolve(c);
-    if (mode == 0 || mode == 1) {
-        k.on = mode == 1;
+    if (mode == 0 || mode == 1 || mode == 3) {
+        k.on = mode != 0;
+        k.mask = mode == 3 ? ((1u << KS_UPDATE32) | (1u << KS_UPDATE64)) : ~0u;
         for (int i = 0; i < KS_COUNT; i++) {
             k.ms[i] = k.flops[i] = k.bytes[i] = 0.0;
             k.launches[i] = 0;

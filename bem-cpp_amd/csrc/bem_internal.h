@@ -61,11 +61,14 @@ enum KernelClass {
     KS_FARFIELD,     // far field partial sums
     KS_RHS,          // plane-wave right-hand sides
     KS_OTHER,
+    KS_UPDATE32,     // trailing-update GEMMs of the LU, FP32 3M (own kernel symbol)
+    KS_UPDATE64,     // trailing-update GEMMs of the LU, FP64
     KS_COUNT
 };
 
 struct KernelTimer {
     bool on = false;
+    unsigned mask = ~0u;  // classes that get events
     struct Rec {
         hipEvent_t a, b;
         hipStream_t s;
@@ -93,6 +96,7 @@ struct GemmPlan {
     long cap = 0;                            // grid of capped launches; 0: from occ and cus
     bool trialed = false;                    // `large` chosen by whole trial factorisations
     double trial_ms[kMaxCfgs] = {0, 0, 0, 0};  // their times per configuration (0: not run)
+    double alone_tflops[kMaxCfgs] = {0, 0, 0, 0};  // trailing-update shape timed alone (autotuner)
     GemmPlan() = default;
     GemmPlan(int l, int s) : large(l), skinny(s) {}
 };
@@ -234,7 +238,7 @@ inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 // (default: the context stream).
 inline int kt_mark(bem_ctx* c, int cls, hipStream_t s = nullptr) {
     KernelTimer& k = c->kt;
-    if (!k.on) return -1;
+    if (!k.on || !((k.mask >> cls) & 1u)) return -1;
     KernelTimer::Rec r{};
     for (hipEvent_t* e : {&r.a, &r.b}) {
         if (!k.pool.empty()) {

@@ -119,7 +119,9 @@ __device__ __forceinline__ void vec_split<double>(const double2& v, double* o) {
     o[0] = v.x; o[1] = v.y;
 }
 
-template <typename T, int CFG>
+// ROLE only names the instantiation: 1 for the LU trailing updates, so a
+// profiler lists them apart from the other products of the same tile.
+template <typename T, int CFG, int ROLE = 0>
 __global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, int N, int K, const T* __restrict__ Ar,
                                                const T* __restrict__ Ai, long lda, const T* __restrict__ Br,
                                                const T* __restrict__ Bi, long ldb, T* __restrict__ Cr,
@@ -304,24 +306,28 @@ __global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, i
 template <typename T, int CFG>
 inline void cgemm_cfg(hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
                       const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode, int nb, long sa, long sb, long sc,
-                      long cap) {
+                      long cap, int role) {
     typedef GemmCfg<T, CFG> C;
     const long tiles = (long)((N + C::BN - 1) / C::BN) * ((M + C::BM - 1) / C::BM) * nb;
     const long grid = cap > 0 && cap < tiles ? cap : tiles;
-    hipLaunchKernelGGL((k_cgemm<T, CFG>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr,
-                       Ci, ldc, mode, nb, sa, sb, sc);
+    if (role == 1)
+        hipLaunchKernelGGL((k_cgemm<T, CFG, 1>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi,
+                           ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc);
+    else
+        hipLaunchKernelGGL((k_cgemm<T, CFG, 0>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi,
+                           ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc);
 }
 
 template <typename T>
 inline void cgemm_any(int cfg, hipStream_t s, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
                       const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode, int nb = 1, long sa = 0, long sb = 0,
-                      long sc = 0, long cap = 0) {
+                      long sc = 0, long cap = 0, int role = 0) {
     if (M <= 0 || N <= 0 || K <= 0 || nb <= 0) return;
     switch (cfg) {
-        case 1: cgemm_cfg<T, 1>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
-        case 2: cgemm_cfg<T, 2>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
-        case 3: cgemm_cfg<T, 3>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
-        default: cgemm_cfg<T, 0>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap); break;
+        case 1: cgemm_cfg<T, 1>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
+        case 2: cgemm_cfg<T, 2>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
+        case 3: cgemm_cfg<T, 3>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
+        default: cgemm_cfg<T, 0>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
     }
 }
 
@@ -378,8 +384,9 @@ inline int gemm_choose(const bem_ctx* c, int M, int N) {
 // Launch flags of cgemm_ctx: GEMM_HEADROOM caps the grid as
 // gemm_headroom_cap does (an update running beside the panel stream);
 // GEMM_PANEL marks a product of the panel stream's critical path: the plan's
-// panel configuration at raised issue priority.
-constexpr int GEMM_HEADROOM = 1, GEMM_PANEL = 2;
+// panel configuration at raised issue priority; GEMM_UPDATE: an LU trailing
+// update (its own kernel symbol and timing class).
+constexpr int GEMM_HEADROOM = 1, GEMM_PANEL = 2, GEMM_UPDATE = 4;
 
 // GEMM with optional per-launch timing, on stream s (default: the context
 // stream), tile configuration from the context's plan.  Counted work: 3M
@@ -390,12 +397,13 @@ inline void cgemm_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai,
                       long ldb, T* Cr, T* Ci, long ldc, int mode, hipStream_t s = nullptr, int flags = 0) {
     if (M <= 0 || N <= 0 || K <= 0) return;
     if (!s) s = c->stream;
-    const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, s);
+    const bool upd = flags & GEMM_UPDATE;
+    const int k = kt_mark(c, sizeof(T) == 4 ? (upd ? KS_UPDATE32 : KS_GEMM32) : (upd ? KS_UPDATE64 : KS_GEMM64), s);
     const GemmPlan& plan = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
     const int cfg = (flags & GEMM_PANEL) ? plan.panel : gemm_choose<T>(c, M, N);
     const long cap = (flags & GEMM_HEADROOM) ? gemm_headroom_cap(plan, cfg) : 0;
     if (flags & GEMM_PANEL) mode |= GEMM_MODE_URGENT;
-    cgemm_any<T>(cfg, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, cap);
+    cgemm_any<T>(cfg, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, cap, upd ? 1 : 0);
     const double mnk = (double)M * N * K;
     const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
     kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);

@@ -109,6 +109,7 @@ int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* r
         return bem_fail(c, BEM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
     plan.large = bl;
     plan.skinny = bs;
+    for (int i = 0; i < kGemmCfgs; i++) plan.alone_tflops[i] = rl[i];
     plan.cap = 0;
     plan.trialed = false;  // the first factorisation of this size times whole trials
     for (double& t : plan.trial_ms) t = 0.0;

@@ -66,14 +66,17 @@ const char* bem_last_error(const bem_ctx* ctx);
  * [0] assembly  [1] factorisation  [2] solve  [3] refinement  [4] far field
  * [5] rhs generation  [6] adjoint.  Returns the number written (<= n). */
 int bem_timings(const bem_ctx* ctx, double* out, int n);
-/* Per-kernel-class device time, measured with HIP events on the context
- * stream around every launch (no counterpart in the reference; used for the
- * roofline figures).  mode 1: reset and enable, 0: reset and disable,
- * 2: read.  For class i, out[4i..4i+3] = launches, total ms, counted flops,
- * counted bytes.  Classes: 0 GEMM FP32-3M, 1 GEMM FP64, 2 assembly far
- * pairs, 3 assembly near pairs, 4 assembly gather, 5 pivot tournament,
- * 6 row swaps, 7 panel, 8 triangular solves, 9 far field, 10 RHS, 11 other.
- * Returns the number of classes. */
+/* Per-kernel-class device time, measured with HIP events on the stream of
+ * each launch (no counterpart in the reference; used for the roofline
+ * figures).  mode 1: reset and enable for every class, 3: reset and enable
+ * for the LU trailing-update GEMMs only (two events per launch cost the
+ * device about 2 us each, so timing every launch slows a factorisation by
+ * ~10 %), 0: reset and disable, 2: read.  For class i, out[4i..4i+3] =
+ * launches, total ms, counted flops, counted bytes.  Classes: 0 GEMM
+ * FP32-3M, 1 GEMM FP64, 2 assembly far pairs, 3 assembly near pairs,
+ * 4 assembly gather, 5 pivot tournament, 6 row swaps, 7 panel, 8 triangular
+ * solves, 9 far field, 10 RHS, 11 other, 12 LU trailing-update GEMM FP32-3M,
+ * 13 LU trailing-update GEMM FP64.  Returns the number of classes. */
 int bem_kernel_stats(bem_ctx* ctx, int mode, double* out, int n);
 
 /* Times the GEMM tile configurations of the factorisation and the solves at

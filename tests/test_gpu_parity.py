@@ -312,7 +312,8 @@ def test_every_gemm_tile_and_capped_grid_factor_exactly(ctx, cfg):
 def test_factor_trials_pick_a_tile_and_keep_results(ctx):
     # from 8192 unknowns on, the first factorisation of each precision times
     # one whole trial factorisation per tile configuration (on the context's
-    # own matrix) and keeps the fastest; the factors are the same either way
+    # own matrix) and keeps the fastest (ties within 3 % go to the tile that
+    # is faster alone); the factors are the same either way
     rs = np.random.RandomState(12)
     n = 8192
     Z = (rs.standard_normal((n, n)) + 1j * rs.standard_normal((n, n))).astype(np.complex128)
@@ -324,7 +325,9 @@ def test_factor_trials_pick_a_tile_and_keep_results(ctx):
         assert info["trialed"], info
         tiles = ctx.GEMM_TILES if prec == "fp32" else ctx.GEMM_TILES64
         assert sorted(info["trial_factor_ms"]) == sorted(tiles), info
-        assert info["large"] == min(info["trial_factor_ms"], key=info["trial_factor_ms"].get)
+        # the fastest trial, or one within 3 % of it that is faster alone
+        ms = info["trial_factor_ms"]
+        assert ms[info["large"]] <= 1.03 * min(ms.values()) + 1e-9, info
         X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
         be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
         # partial pivoting's growth on a random matrix rises with n: at
@@ -333,3 +336,24 @@ def test_factor_trials_pick_a_tile_and_keep_results(ctx):
         # a second factorisation reuses the choice (no further trials)
         assert ctx.factor(prec) == 0
         assert ctx.gemm_plan_info(prec) == info
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_overlapped_streams_match_one_stream_bitwise(bem, prec, monkeypatch):
+    # the three-stream factorisation applies every update in the same order
+    # as the one-stream one, so the solutions are bitwise equal; a missing
+    # cross-stream dependency shows up as a difference.  The 7-block grid of
+    # the trailing updates stretches the overlap.
+    rs = np.random.RandomState(13)
+    n = 1800
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n)
+    B = rs.randn(n, 6) + 1j * rs.randn(n, 6)
+    X = {}
+    for layout in ("0", "1"):
+        monkeypatch.setenv("BEM_LU_STREAMS", layout)
+        with bem.Context(0) as c:
+            c.set_Z(Z)
+            c.gemm_plan(prec, 2, 2, cap=7)
+            assert c.factor(prec) == 0
+            X[layout], _ = c.solve(B, ir_max=0)
+    assert np.array_equal(X["0"], X["1"])
