@@ -119,8 +119,11 @@ __device__ __forceinline__ void vec_split<double>(const double2& v, double* o) {
     o[0] = v.x; o[1] = v.y;
 }
 
-// ROLE only names the instantiation: 1 for the LU trailing updates, so a
-// profiler lists them apart from the other products of the same tile.
+// ROLE 1 only names the instantiation (the LU trailing updates, so a
+// profiler lists them apart from the other products of the same tile).
+// ROLE 2 selects the 3M form in FP64: the refinement residual, whose
+// rounding (a few eps of |Z||X|) only has to stay far below the FP32
+// factors' error, takes three real products instead of four.
 template <typename T, int CFG, int ROLE = 0>
 __global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, int N, int K, const T* __restrict__ Ar,
                                                const T* __restrict__ Ai, long lda, const T* __restrict__ Br,
@@ -135,7 +138,8 @@ __global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, i
     constexpr int AV = (AVT + 255) / 256, BV = (BVT + 255) / 256;  // per thread
     typedef typename C::vec_t vec_t;
     typedef typename C::acc_t acc_t;
-    constexpr int NACC = sizeof(T) == 4 ? 3 : 2;
+    constexpr bool M3 = sizeof(T) == 4 || ROLE == 2;  // 3M form (FP32 always; FP64 for ROLE 2)
+    constexpr int NACC = M3 ? 3 : 2;
 
     __shared__ T sA[2][2][BK][SA];
     __shared__ T sB[2][2][BK][SB];
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, i
                     br[j] = sB[buf][0][ka][wn0 + j * 16 + (lane & 15)];
                     bi[j] = sB[buf][1][ka][wn0 + j * 16 + (lane & 15)];
                 }
-                if constexpr (sizeof(T) == 4) {
+                if constexpr (M3) {
                     T as[TM], bs[TN];
 #pragma unroll
                     for (int i = 0; i < TM; i++) as[i] = ar[i] + ai[i];
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(256, (GemmCfg<T, CFG>::MINB)) void k_cgemm(int M, i
                     const int row = row0 + wm0 + i * 16 + (sizeof(T) == 4 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r);
                     if (row >= M || col >= N) continue;
                     T vr, vi;
-                    if constexpr (sizeof(T) == 4) {
+                    if constexpr (M3) {
                         vr = acc[0][i][j][r] - acc[1][i][j][r];
                         vi = acc[2][i][j][r] - acc[0][i][j][r] - acc[1][i][j][r];
                     } else {
@@ -313,6 +317,14 @@ inline void cgemm_cfg(hipStream_t s, int M, int N, int K, const T* Ar, const T* 
     if (role == 1)
         hipLaunchKernelGGL((k_cgemm<T, CFG, 1>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi,
                            ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc);
+    else if constexpr (sizeof(T) == 8 && CFG != 0) {  // the 128 x 64 tile would spill in the 3M form
+        if (role == 2)
+            hipLaunchKernelGGL((k_cgemm<T, CFG, 2>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br,
+                               Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc);
+        else
+            hipLaunchKernelGGL((k_cgemm<T, CFG, 0>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br,
+                               Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc);
+    }
     else
         hipLaunchKernelGGL((k_cgemm<T, CFG, 0>), dim3((unsigned)grid), dim3(256), 0, s, M, N, K, Ar, Ai, lda, Br, Bi,
                            ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc);
@@ -461,13 +473,16 @@ inline int gemm_splits(bem_ctx* c, int cfg, int M, int N, int K) {
 // Falls back to one product when splitting does not pay.
 template <typename T>
 inline int cgemm_splitk_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
-                            const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
+                            const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode, int role = 0) {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     gemm_plan_occupancy(c, sizeof(T) == 4);
     const int cfg = gemm_choose<T>(c, M, N);
     const int S = gemm_splits<T>(c, cfg, M, N, K);
+    const double fl = (sizeof(T) == 4 || (role == 2 && cfg != 0) ? 6.0 : 8.0) * (double)M * N * K;
     if (S == 1) {
-        cgemm_ctx<T>(c, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+        const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, c->stream);
+        cgemm_any<T>(cfg, c->stream, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, 0, role);
+        kt_close(c, k, fl, 2.0 * sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N));
         return 0;
     }
     const long count = (long)M * N;
@@ -483,11 +498,10 @@ inline int cgemm_splitk_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const 
     const int Ks = K / S;
     const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, c->stream);
     cgemm_any<T>(cfg, c->stream, M, N, Ks, Ar, Ai, lda, Br, Bi, ldb, P, P + count, N, 1, S, (long)Ks, (long)Ks * ldb,
-                 2 * count);
+                 2 * count, 0, role);
     hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, count, N, S,
                        (const T*)P, Cr, Ci, ldc, mode);
-    const double mnk = (double)M * N * K;
     const double b = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (2.0 * S + (mode == 1 ? 1.0 : 2.0)) * count);
-    kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, b);
+    kt_close(c, k, fl, b);
     return 0;
 }

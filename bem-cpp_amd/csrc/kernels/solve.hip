@@ -9,7 +9,8 @@
 // matrix cores (gemm.h), for all right-hand sides at once.
 //
 // Mixed precision (factors in FP32): X_0 = 0, R_0 = B, and per sweep
-//   D = solve32(fp32(P R));  X += D;  R = B - Z X   (FP64 GEMM with the FP64 Z)
+//   D = solve32(fp32(P R));  X += D;  R = B - Z X   (FP64 GEMM with the FP64 Z,
+//   3M form, K split over batched partial products)
 // until max_j ||R_j|| / ||B_j|| <= tol.  Norm reductions use a fixed order.
 #include "../bem_internal.h"
 #include "gemm.h"
@@ -157,7 +158,8 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
     auto residual = [&]() -> int {  // R = B - Z X, returns max relative residual in *res
         BEM_HIP(c, hipMemcpyAsync(Rr, c->Br, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
         BEM_HIP(c, hipMemcpyAsync(Ri, c->Bi, cnt * sizeof(double), hipMemcpyDeviceToDevice, s));
-        if (int rc = cgemm_splitk_ctx<double>(c, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0))
+        // 3M form: three real products (see k_cgemm, ROLE 2)
+        if (int rc = cgemm_splitk_ctx<double>(c, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0, 2))
             return rc;
         hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, s, c->n, Rr, Ri, (long)ldr, nr2);
         BEM_HIP(c, hipMemcpyAsync(r2.data(), nr2, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--gemm-tile", default=None,
+                    help="fix the FP32 GEMM tile (e.g. 64x64) instead of the trial factorisations; for counter runs, "
+                         "whose serialised kernels distort the trials")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="time the steps without the per-launch HIP events (no per-kernel classes or roofline)")
     ap.add_argument("--skip-external", action="store_true",
@@ -177,6 +180,9 @@ def main():
         log(f"copy calibration (k_copy16): {args.copy_calibration} MiB read + {args.copy_calibration} MiB written")
     N = ctx.set_mesh(verts, tris, 7)
     n = 2 * N
+    if args.gemm_tile:
+        t = ctx.GEMM_TILES.index(args.gemm_tile)
+        ctx.gemm_plan("fp32", t, t)
     log(f"workload {args.workload}: {shape} ref {ref} T={len(tris)} N={N} 2N={n} ka={ka} m={m} "
         f"orient={n_orient} rhs={nrhs}")
 
