@@ -275,8 +275,8 @@ class Context:
         return g
 
     # GEMM configurations (block tile; "k8": K slice of 8 instead of 16)
-    GEMM_TILES = ("128x128", "128x64", "64x64", "64x64k8")  # FP32
-    GEMM_TILES64 = ("128x64", "64x64", "64x32", "64x64k8")  # FP64
+    GEMM_TILES = ("128x128", "128x64", "64x64", "64x64k8", "32x32")  # FP32
+    GEMM_TILES64 = ("128x64", "64x64", "64x32", "64x64k8", "32x32")  # FP64
 
     def autotune(self, nrhs=128):
         """Times the GEMM tile configurations at this system's shapes and keeps

@@ -84,19 +84,19 @@ struct KernelTimer {
 // GEMM tile choice of one precision (configurations of kernels/gemm.h):
 // `large` for products with at least `fill` blocks in it, else `skinny`.
 struct GemmPlan {
-    static constexpr int kMaxCfgs = 4;
+    static constexpr int kMaxCfgs = 5;
     int large = 0, skinny = 2;
     int panel = 3;                           // products of the panel stream (smallest LDS footprint)
     long fill = 512;
     bool tuned = false;
     int npad = 0;                            // system size the plan was tuned for
     int cus = 256;                           // compute units of the device
-    int occ[kMaxCfgs] = {1, 1, 1, 1};        // resident blocks per CU of each configuration
+    int occ[kMaxCfgs] = {1, 1, 1, 1, 1};     // resident blocks per CU of each configuration
     bool occ_known = false;                  // cus and occ queried from the device
     long cap = 0;                            // grid of capped launches; 0: from occ and cus
     bool trialed = false;                    // `large` chosen by whole trial factorisations
-    double trial_ms[kMaxCfgs] = {0, 0, 0, 0};  // their times per configuration (0: not run)
-    double alone_tflops[kMaxCfgs] = {0, 0, 0, 0};  // trailing-update shape timed alone (autotuner)
+    double trial_ms[kMaxCfgs] = {};          // their times per configuration (0: not run)
+    double alone_tflops[kMaxCfgs] = {};      // trailing-update shape timed alone (autotuner)
     GemmPlan() = default;
     GemmPlan(int l, int s) : large(l), skinny(s) {}
 };

@@ -106,7 +106,26 @@ struct GemmCfg<double, 3> {
     typedef double2 vec_t;
     static constexpr int VEC = 2;
 };
-constexpr int kGemmCfgs = 4;
+// 32 x 32 (one 16 x 16 MFMA tile per wave): products too small to give the
+// device one block per CU with the other tiles (the block-inverse products
+// and the late updates of the solves, the L11^{-1} doubling).
+template <>
+struct GemmCfg<float, 4> {
+    static constexpr int BM = 32, BN = 32, BK = 16, WM = 16, WN = 16, PAD = 16, MINB = 4;
+    typedef v4f acc_t;
+    typedef float4 vec_t;
+    static constexpr int VEC = 4;
+};
+template <>
+struct GemmCfg<double, 4> {
+    static constexpr int BM = 32, BN = 32, BK = 16, WM = 16, WN = 16, PAD = 8, MINB = 4;
+    typedef v4d acc_t;
+    typedef double2 vec_t;
+    static constexpr int VEC = 2;
+};
+constexpr int kGemmCfgs = 5;
+constexpr int kGemmSmall = 4;       // the 32 x 32 configuration
+constexpr int kGemmTrialCfgs = 4;   // configurations tried for the trailing update
 
 template <typename T>
 __device__ __forceinline__ void vec_split(const typename GemmCfg<T, 0>::vec_t& v, T* out);
@@ -339,6 +358,7 @@ inline void cgemm_any(int cfg, hipStream_t s, int M, int N, int K, const T* Ar, 
         case 1: cgemm_cfg<T, 1>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
         case 2: cgemm_cfg<T, 2>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
         case 3: cgemm_cfg<T, 3>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
+        case 4: cgemm_cfg<T, 4>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
         default: cgemm_cfg<T, 0>(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, nb, sa, sb, sc, cap, role); break;
     }
 }
@@ -358,6 +378,7 @@ inline int cgemm_occupancy(int cfg) {
         case 1: return cgemm_occ_cfg<T, 1>();
         case 2: return cgemm_occ_cfg<T, 2>();
         case 3: return cgemm_occ_cfg<T, 3>();
+        case 4: return cgemm_occ_cfg<T, 4>();
         default: return cgemm_occ_cfg<T, 0>();
     }
 }
@@ -380,17 +401,20 @@ inline long gemm_blocks(int cfg, int M, int N) {
         case 1: bm = GemmCfg<T, 1>::BM; bn = GemmCfg<T, 1>::BN; break;
         case 2: bm = GemmCfg<T, 2>::BM; bn = GemmCfg<T, 2>::BN; break;
         case 3: bm = GemmCfg<T, 3>::BM; bn = GemmCfg<T, 3>::BN; break;
+        case 4: bm = GemmCfg<T, 4>::BM; bn = GemmCfg<T, 4>::BN; break;
         default: bm = GemmCfg<T, 0>::BM; bn = GemmCfg<T, 0>::BN; break;
     }
     return (long)((M + bm - 1) / bm) * ((N + bn - 1) / bn);
 }
 
 // The plan's configuration for an M x N product: the large-product tile when
-// it still gives at least `fill` blocks, else the skinny-product tile.
+// it still gives at least `fill` blocks, else the skinny-product tile while
+// that gives one block per CU, else the 32 x 32 tile.
 template <typename T>
 inline int gemm_choose(const bem_ctx* c, int M, int N) {
     const GemmPlan& p = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
-    return gemm_blocks<T>(p.large, M, N) >= p.fill ? p.large : p.skinny;
+    if (gemm_blocks<T>(p.large, M, N) >= p.fill) return p.large;
+    return gemm_blocks<T>(p.skinny, M, N) >= p.cus ? p.skinny : kGemmSmall;
 }
 
 // Launch flags of cgemm_ctx: GEMM_HEADROOM caps the grid as

@@ -563,8 +563,8 @@ void launch_laswp(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, long ld, int ca, int 
 // LDS substitution launch; then for s = 32, 64, 128 every pair of s x s
 // diagonal blocks is joined,
 //     [X11 0; X21 X22] with X21 = -X22 (L21 X11),
-// as two batched GEMMs (all pairs of a level in one launch each, on the tile
-// with the smallest LDS footprint, since they run beside the trailing update).  The blocks
+// as two batched GEMMs (all pairs of a level in one launch each, on the 32 x 32
+// tile: many blocks, little LDS, since they run beside the trailing update).  The blocks
 // above the diagonal are not written: the caller keeps them zero.  Seven
 // launches with short dependency chains replace one 256-step substitution.
 template <typename T>
@@ -578,10 +578,10 @@ void invert_lower_doubling(hipStream_t s, const T* Fr, const T* Fi, long ldf, T*
         const long sf = 2L * h * (ldf + 1), si = 2L * h * (NB + 1), st = (long)h * h;
         const long f21 = (long)h * ldf, i21 = (long)h * NB, i22 = (long)h * (NB + 1);
         // T = L21 X11, then X21 = -X22 T
-        cgemm_any<T>(3, s, h, h, h, Fr + f21, Fi + f21, ldf, Ir, Ii, NB, w.tr, w.ti, h, 1 | GEMM_MODE_URGENT, pairs,
-                     sf, si, st);
-        cgemm_any<T>(3, s, h, h, h, Ir + i22, Ii + i22, NB, w.tr, w.ti, h, Ir + i21, Ii + i21, NB, 3 | GEMM_MODE_URGENT,
-                     pairs, si, st,
+        cgemm_any<T>(kGemmSmall, s, h, h, h, Fr + f21, Fi + f21, ldf, Ir, Ii, NB, w.tr, w.ti, h, 1 | GEMM_MODE_URGENT,
+                     pairs, sf, si, st);
+        cgemm_any<T>(kGemmSmall, s, h, h, h, Ir + i22, Ii + i22, NB, w.tr, w.ti, h, Ir + i21, Ii + i21, NB,
+                     3 | GEMM_MODE_URGENT, pairs, si, st,
                      si);
     }
 }
@@ -854,7 +854,7 @@ int factor_trials(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes, bool*
     }
     *ran = true;
     int rc = 0;
-    for (int cfg = 0; cfg < kGemmCfgs && !rc; cfg++) {
+    for (int cfg = 0; cfg < kGemmTrialCfgs && !rc; cfg++) {
         plan.large = cfg;
         if ((rc = load_working_copy<T>(c, Ar, Ai))) break;
         (void)hipEventRecord(e0, c->stream);
@@ -873,11 +873,11 @@ int factor_trials(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes, bool*
     // the fastest trial; among trials within 3 % of it (run-to-run noise),
     // the configuration with the higher rate timed alone
     int best = 0;
-    for (int cfg = 1; cfg < kGemmCfgs; cfg++)
+    for (int cfg = 1; cfg < kGemmTrialCfgs; cfg++)
         if (plan.trial_ms[cfg] < plan.trial_ms[best]) best = cfg;
     const double lim = 1.03 * plan.trial_ms[best];
     int pick = best;
-    for (int cfg = 0; cfg < kGemmCfgs; cfg++)
+    for (int cfg = 0; cfg < kGemmTrialCfgs; cfg++)
         if (plan.trial_ms[cfg] <= lim && plan.alone_tflops[cfg] > plan.alone_tflops[pick]) pick = cfg;
     plan.large = pick;
     if (rc) return rc;

@@ -85,7 +85,7 @@ int bem_kernel_stats(bem_ctx* ctx, int mode, double* out, int n);
  * factorisation also tunes itself on its first call for a size unless
  * BEM_AUTOTUNE=0).  out (nullable, nout doubles) receives the measured TFLOP/s: FP32
  * trailing-update shape per configuration, FP32 solve shape, FP64
- * trailing-update shape, FP64 solve shape (4 configurations each), then the
+ * trailing-update shape, FP64 solve shape (5 configurations each), then the
  * chosen (large, skinny) configuration per precision.  Returns the number of
  * values written, or < 0. */
 int bem_autotune(bem_ctx* ctx, int nrhs, double* out, int nout);

@@ -286,7 +286,7 @@ def test_autotune_reports_and_factorisation_stays_exact(ctx):
         assert be < BE_FP64
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
 def test_every_gemm_tile_and_capped_grid_factor_exactly(ctx, cfg):
     # each tile configuration as the only one, with the capped trailing
     # updates on a 7-block grid, so every block loops over several tiles and
@@ -324,7 +324,7 @@ def test_factor_trials_pick_a_tile_and_keep_results(ctx):
         info = ctx.gemm_plan_info(prec)
         assert info["trialed"], info
         tiles = ctx.GEMM_TILES if prec == "fp32" else ctx.GEMM_TILES64
-        assert sorted(info["trial_factor_ms"]) == sorted(tiles), info
+        assert sorted(info["trial_factor_ms"]) == sorted(tiles[:4]), info  # the 32 x 32 tile is not tried
         # the fastest trial, or one within 3 % of it that is faster alone
         ms = info["trial_factor_ms"]
         assert ms[info["large"]] <= 1.03 * min(ms.values()) + 1e-9, info
