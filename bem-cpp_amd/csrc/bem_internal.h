@@ -94,6 +94,8 @@ struct GemmPlan {
     int occ[kMaxCfgs] = {1, 1, 1, 1, 1};     // resident blocks per CU of each configuration
     bool occ_known = false;                  // cus and occ queried from the device
     long cap = 0;                            // grid of capped launches; 0: from occ and cus
+    int sk_cfg = -1;                         // split-K products: configuration timed fastest for
+    int sk_shape[4] = {0, 0, 0, 0};          //   this (M, N, K, role)
     bool trialed = false;                    // `large` chosen by whole trial factorisations
     double trial_ms[kMaxCfgs] = {};          // their times per configuration (0: not run)
     double alone_tflops[kMaxCfgs] = {};      // trailing-update shape timed alone (autotuner)

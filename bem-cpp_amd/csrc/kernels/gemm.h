@@ -28,6 +28,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 constexpr int GEMM_MODE_URGENT = 4;
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -492,25 +494,39 @@ inline int gemm_splits(bem_ctx* c, int cfg, int M, int N, int K) {
     return best;
 }
 
-// C = C - A B (mode 0) or C = A B (mode 1) with K split over S batched
-// products into the context's partial buffer, then summed (k_splitk_sum).
-// Falls back to one product when splitting does not pay.
+// One split-K product on configuration cfg with S splits: S batched partial
+// products into P (2 S M N values), summed into C by k_splitk_sum.
+template <typename T>
+inline void splitk_launch(bem_ctx* c, int cfg, int S, int M, int N, int K, const T* Ar, const T* Ai, long lda,
+                          const T* Br, const T* Bi, long ldb, T* P, T* Cr, T* Ci, long ldc, int mode, int role) {
+    if (S == 1) {
+        cgemm_any<T>(cfg, c->stream, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, 0, role);
+        return;
+    }
+    const long count = (long)M * N;
+    const int Ks = K / S;
+    cgemm_any<T>(cfg, c->stream, M, N, Ks, Ar, Ai, lda, Br, Bi, ldb, P, P + count, N, 1, S, (long)Ks, (long)Ks * ldb,
+                 2 * count, 0, role);
+    hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, count, N, S,
+                       (const T*)P, Cr, Ci, ldc, mode);
+}
+
+// C = C - A B (mode 0) or C = A B (mode 1) for products with few output tiles
+// and a long K (the refinement residual), with K split over S batched
+// products into the context's partial buffer, then summed in split order
+// (k_splitk_sum).  The configuration is chosen once per shape by timing
+// each one (with its own S) into a scratch output; the choice changes speed
+// only.
 template <typename T>
 inline int cgemm_splitk_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
                             const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode, int role = 0) {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     gemm_plan_occupancy(c, sizeof(T) == 4);
-    const int cfg = gemm_choose<T>(c, M, N);
-    const int S = gemm_splits<T>(c, cfg, M, N, K);
-    const double fl = (sizeof(T) == 4 || (role == 2 && cfg != 0) ? 6.0 : 8.0) * (double)M * N * K;
-    if (S == 1) {
-        const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, c->stream);
-        cgemm_any<T>(cfg, c->stream, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, 0, role);
-        kt_close(c, k, fl, 2.0 * sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N));
-        return 0;
-    }
+    GemmPlan& plan = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
     const long count = (long)M * N;
-    const size_t bytes = 2 * (size_t)S * count * sizeof(T);
+    int Smax = 1;
+    for (int cfg = 0; cfg < kGemmCfgs; cfg++) Smax = std::max(Smax, gemm_splits<T>(c, cfg, M, N, K));
+    const size_t bytes = (2 * (size_t)Smax + 2) * count * sizeof(T);  // partials + a scratch output
     if (c->part_bytes < bytes) {
         if (c->part) (void)hipFree(c->part);
         c->part = nullptr;
@@ -519,13 +535,43 @@ inline int cgemm_splitk_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const 
         c->part_bytes = bytes;
     }
     T* P = static_cast<T*>(c->part);
-    const int Ks = K / S;
-    const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, c->stream);
-    cgemm_any<T>(cfg, c->stream, M, N, Ks, Ar, Ai, lda, Br, Bi, ldb, P, P + count, N, 1, S, (long)Ks, (long)Ks * ldb,
-                 2 * count, 0, role);
-    hipLaunchKernelGGL(k_splitk_sum<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, count, N, S,
-                       (const T*)P, Cr, Ci, ldc, mode);
+    if (plan.sk_cfg < 0 || plan.sk_shape[0] != M || plan.sk_shape[1] != N || plan.sk_shape[2] != K ||
+        plan.sk_shape[3] != role) {
+        T* Xr = P + 2 * (size_t)Smax * count;  // scratch output, ld N
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            if (e0) (void)hipEventDestroy(e0);
+            return bem_fail(c, BEM_ERR_HIP, "split-K tuning: events");
+        }
+        float best_ms = 0.0f;
+        int best = gemm_choose<T>(c, M, N);
+        for (int cfg = 0; cfg < kGemmCfgs; cfg++) {
+            const int S = gemm_splits<T>(c, cfg, M, N, K);
+            (void)hipEventRecord(e0, c->stream);
+            splitk_launch<T>(c, cfg, S, M, N, K, Ar, Ai, lda, Br, Bi, ldb, P, Xr, Xr + count, N, 1, role);
+            (void)hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms > 0.0f && (best_ms == 0.0f || ms < best_ms)) {
+                best_ms = ms;
+                best = cfg;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        plan.sk_cfg = best;
+        plan.sk_shape[0] = M;
+        plan.sk_shape[1] = N;
+        plan.sk_shape[2] = K;
+        plan.sk_shape[3] = role;
+    }
+    const int cfg = plan.sk_cfg;
+    const int S = gemm_splits<T>(c, cfg, M, N, K);
+    const double fl = (sizeof(T) == 4 || (role == 2 && cfg != 0) ? 6.0 : 8.0) * (double)M * N * K;
     const double b = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (2.0 * S + (mode == 1 ? 1.0 : 2.0)) * count);
+    const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, c->stream);
+    splitk_launch<T>(c, cfg, S, M, N, K, Ar, Ai, lda, Br, Bi, ldb, P, Cr, Ci, ldc, mode, role);
     kt_close(c, k, fl, b);
     return 0;
 }
