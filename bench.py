@@ -268,7 +268,10 @@ def main():
     if launches > 0 and kms > 0:
         ach = kflops / (kms * 1e-3) / 1e12
         tile = plan32["large"] if plan32 else "?"
-        roof = {"kernel": f"k_cgemm<float, {tile} tile, 1> (3M complex GEMM, LU trailing updates)", "bound": "mfma",
+        small = plan32["skinny"] if plan32 else "?"
+        roof = {"kernel": f"k_cgemm<float, cfg, 1> (3M complex GEMM, LU trailing updates: {tile} tile; "
+                          f"{small} / 32x32 for the late updates that {tile} cannot spread over all CUs)",
+                "bound": "mfma",
                 "achieved": ach, "peak": PEAK_TFLOPS["gemm_fp32_3m"], "unit": "TFLOP/s",
                 "frac": ach / PEAK_TFLOPS["gemm_fp32_3m"], "traffic": measured_traffic(args.workload, dom),
                 "launches": int(launches), "avg_launch_ms": kms / launches,
