@@ -74,6 +74,8 @@ def lib():
         "bem_gemm_plan": ([vp, i, i, i, ctypes.c_long], i), "bem_gemm_plan_get": ([vp, i, _dp, i], i),
         "bem_mesh_counts": ([i, i, _ip, _ip], i), "bem_make_mesh": ([i, i, d, _dp, _ip], i),
         "bem_orientations": ([i, i, i, i, i, i, i, _dp, _dp], i),
+        "bem_write_json": ([ctypes.c_char_p, _dp, _dp, i, d, d, d, i, i, i, i, _dp], i),
+        "bem_write_per_orient": ([ctypes.c_char_p, i, i, i, i, _dp, _dp], i),
         "bem_set_mesh": ([vp, _dp, i, _ip, i, i], i), "bem_num_rwg": ([vp], i),
         "bem_get_rwg": ([vp, _ip, _ip, _dp, _dp, _dp, _dp, _dp], i),
         "bem_assemble": ([vp, d, d, d, d, d, d, u], i), "bem_get_Z": ([vp, _dp, i], i),
@@ -119,6 +121,33 @@ def orientations(na, nb, ng, beta_sym=1, gamma_sym=1, mirror=False, sort=True):
     if got < 0:
         raise BemError(got, "bem_orientations")
     return RT[:got], w[:got]
+
+
+def write_json(path, M, theta, ka, n_re, n_im, refinements, orient=(1, 1, 1), timing=(0.0, 0.0, 0.0, 0.0)):
+    """The reference's result file (output.cpp:5-64) through the library's
+    writer: M is 4 x 4 x ntheta, theta in radians, timing = (assembly, solve,
+    far field, total) seconds."""
+    M = _cd(M).reshape(16, -1)
+    th = _cd(theta)
+    if M.shape[1] != th.size:
+        raise ValueError("M must be 4 x 4 x len(theta)")
+    tm = _cd(timing)
+    rc = lib().bem_write_json(os.fsencode(path), _p(M), _p(th), th.size, float(ka), float(n_re), float(n_im),
+                              int(refinements), int(orient[0]), int(orient[1]), int(orient[2]), _p(tm))
+    if rc:
+        raise BemError(rc, f"bem_write_json({path})")
+
+
+def write_per_orient(path, weights, M_orient, n_alpha, n_beta):
+    """The per-orientation Mueller file (main.cpp:640-653): weights (n) and
+    M_orient (n x 4 x 4 x ntheta or n x 16 x ntheta), without the 1/|k|^2 of
+    the average."""
+    w = _cd(weights).ravel()
+    Mo = _cd(M_orient).reshape(w.size, 16, -1)
+    rc = lib().bem_write_per_orient(os.fsencode(path), w.size, Mo.shape[2], int(n_alpha), int(n_beta), _p(w),
+                                    _p(Mo))
+    if rc:
+        raise BemError(rc, f"bem_write_per_orient({path})")
 
 
 class Context:

@@ -329,6 +329,21 @@ int bem_make_mesh(int shape, int refinements, double param, double* verts, int* 
     return 0;
 }
 
+int bem_write_json(const char* path, const double* M, const double* theta, int ntheta, double ka, double n_re,
+                   double n_im, int refinements, int n_alpha, int n_beta, int n_gamma, const double* timing) {
+    if (!path || !M || !theta || ntheta < 1 || !timing) return BEM_ERR_ARG;
+    return write_json(path, M, theta, ntheta, ka, n_re, n_im, refinements, n_alpha, n_beta, n_gamma, timing[0],
+                      timing[1], timing[2], timing[3])
+               ? 0
+               : BEM_ERR_ARG;
+}
+
+int bem_write_per_orient(const char* path, int n_orient, int ntheta, int n_alpha, int n_beta, const double* weights,
+                         const double* M_orient) {
+    if (!path || n_orient < 0 || ntheta < 1 || (n_orient > 0 && (!weights || !M_orient))) return BEM_ERR_ARG;
+    return write_per_orient(path, n_orient, ntheta, n_alpha, n_beta, weights, M_orient) ? 0 : BEM_ERR_ARG;
+}
+
 int bem_orientations(int na, int nb, int ng, int bsym, int gsym, int mirror, int sort, double* RT, double* w) {
     if (na < 1 || nb < 1 || ng < 1 || (bsym != 1 && bsym != 2) || gsym < 1 || !RT || !w) return BEM_ERR_ARG;
     std::vector<Orientation> o = generate_orientations(na, nb, ng, bsym, gsym, mirror != 0);

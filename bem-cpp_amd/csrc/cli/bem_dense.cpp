@@ -241,10 +241,14 @@ int main(int argc, char** argv) {
     const double time_solve = ts.elapsed_s();
 
     Timer tf;
-    std::vector<double> M(16 * (size_t)ntheta);
+    std::vector<double> M(16 * (size_t)ntheta), Mo(16 * (size_t)ntheta * no);
     rc = bem_orient_mueller(c, k_ext.real(), k_ext.imag(), eta_ext, RT.data(), w.data(), no, theta.data(), ntheta,
-                            gamma_mirror ? 1 : 0, M.data(), nullptr);
+                            gamma_mirror ? 1 : 0, M.data(), Mo.data());
     if (rc) return fail(c, "far field", rc);
+    // the per-orientation file holds the matrices before the 1/|k|^2 of the
+    // average (main.cpp:554-569, 640-653)
+    const double k2 = std::norm(k_ext);
+    for (double& v : Mo) v *= k2;
     const double time_farfield = tf.elapsed_s();
     double dev[8];
     bem_timings(c, dev, 8);
@@ -253,8 +257,12 @@ int main(int argc, char** argv) {
     bem_destroy(c);
 
     const double time_total = total.elapsed_s();
-    write_json(outfile, M.data(), theta.data(), ntheta, ka, n_re, n_im, refinements, single ? 1 : n_alpha,
-               single ? 1 : n_beta, single ? 1 : n_gamma, time_assembly, time_solve, time_farfield, time_total);
+    const int na_out = single ? 1 : n_alpha, nb_out = single ? 1 : n_beta, ng_out = single ? 1 : n_gamma;
+    if (!write_json(outfile, M.data(), theta.data(), ntheta, ka, n_re, n_im, refinements, na_out, nb_out, ng_out,
+                    time_assembly, time_solve, time_farfield, time_total))
+        return 1;
+    const std::string po_path = std::string(outfile) + ".per_orient";
+    if (!write_per_orient(po_path.c_str(), no, ntheta, na_out, nb_out, w.data(), Mo.data())) return 1;
     std::printf("\n=== Done ===\n  Assembly: %.1fs\n  Solve: %.1fs\n  Farfield: %.1fs\n  Total: %.1fs\n", time_assembly,
                 time_solve, time_farfield, time_total);
     return 0;

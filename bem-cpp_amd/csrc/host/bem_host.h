@@ -127,9 +127,13 @@ std::vector<Orientation> generate_orientations(int n_alpha, int n_beta, int n_ga
 void sort_orientations_nearest(std::vector<Orientation>& orients);
 
 // ---- output ----------------------------------------------------------------
-void write_json(const char* filename, const double* M, const double* theta, int ntheta, double ka, double n_re,
+bool write_json(const char* filename, const double* M, const double* theta, int ntheta, double ka, double n_re,
                 double n_im, int refinements, int n_alpha, int n_beta, int n_gamma, double time_assembly,
                 double time_solve, double time_farfield, double time_total);
+// The per-orientation Mueller file of main.cpp:640-653; returns false when
+// the file cannot be written.
+bool write_per_orient(const char* filename, int n_orient, int ntheta, int n_alpha, int n_beta,
+                      const double* weights, const double* M_orient);
 
 // ---- dense path over the device library (dense_api.cpp) --------------------
 void assemble_pmchwt(const RWG& rwg, const Mesh& mesh, cdouble k_ext, cdouble k_int, double eta_ext,

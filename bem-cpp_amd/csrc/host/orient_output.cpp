@@ -4,6 +4,7 @@
 //   generate_orientations      orient.cpp:73-124
 //   sort_orientations_nearest  orient.cpp:127-163 (greedy Frobenius nearest neighbour)
 //   write_json                 output.cpp:5-64 (byte-compatible format)
+//   write_per_orient           main.cpp:640-653 (the .per_orient binary)
 #include <cmath>
 #include <cstdio>
 
@@ -126,13 +127,13 @@ void sort_orientations_nearest(std::vector<Orientation>& orients) {
     orients.swap(path);
 }
 
-void write_json(const char* filename, const double* M, const double* theta, int ntheta, double ka, double n_re,
+bool write_json(const char* filename, const double* M, const double* theta, int ntheta, double ka, double n_re,
                 double n_im, int refinements, int n_alpha, int n_beta, int n_gamma, double time_assembly,
                 double time_solve, double time_farfield, double time_total) {
     FILE* f = std::fopen(filename, "w");
     if (!f) {
         std::fprintf(stderr, "Cannot open %s for writing\n", filename);
-        return;
+        return false;
     }
     std::fprintf(f, "{\n  \"ka\": %.6f,\n  \"ri\": [%.6f, %.6f],\n  \"refinements\": %d,\n", ka, n_re, n_im,
                  refinements);
@@ -154,6 +155,25 @@ void write_json(const char* filename, const double* M, const double* theta, int 
         std::fprintf(f, i < 3 ? "    ],\n" : "    ]\n");
     }
     std::fprintf(f, "  ]\n}\n");
-    std::fclose(f);
+    const bool ok = !std::ferror(f);
+    if (std::fclose(f) != 0 || !ok) return false;
     std::printf("Results written to %s\n", filename);
+    return true;
+}
+
+bool write_per_orient(const char* filename, int n_orient, int ntheta, int n_alpha, int n_beta,
+                      const double* weights, const double* M_orient) {
+    FILE* f = std::fopen(filename, "wb");
+    if (!f) {
+        std::fprintf(stderr, "Cannot open %s for writing\n", filename);
+        return false;
+    }
+    const int header[4] = {n_orient, ntheta, n_alpha, n_beta};
+    const size_t nm = (size_t)n_orient * 16 * ntheta;
+    bool ok = std::fwrite(header, sizeof(int), 4, f) == 4;
+    ok = ok && std::fwrite(weights, sizeof(double), (size_t)n_orient, f) == (size_t)n_orient;
+    ok = ok && std::fwrite(M_orient, sizeof(double), nm, f) == nm;
+    ok = (std::fclose(f) == 0) && ok;
+    if (ok) std::printf("  Per-orient Mueller saved: %s (%d orient x 16 x %d)\n", filename, n_orient, ntheta);
+    return ok;
 }

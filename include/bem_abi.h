@@ -126,6 +126,22 @@ int bem_make_mesh(int shape, int refinements, double param, double* verts, int* 
 int bem_orientations(int n_alpha, int n_beta, int n_gamma, int beta_sym, int gamma_sym, int gamma_mirror,
                      int sort, double* RT, double* w);
 
+/* ---- output files (no device work) -------------------------------------- */
+/* Replaces write_json (output.h:6-14, output.cpp:5-64): the result file of
+ * the reference, byte for byte (M is 4 x 4 x ntheta, [(i*4+j)*ntheta + t];
+ * theta in radians; timing = {assembly, solve, far field, total} seconds).
+ * Returns 0, or BEM_ERR_ARG when the file cannot be written. */
+int bem_write_json(const char* path, const double* M, const double* theta, int ntheta, double ka, double n_re,
+                   double n_im, int refinements, int n_alpha, int n_beta, int n_gamma, const double* timing);
+/* Replaces the per-orientation Mueller file of main.cpp:640-653
+ * (`<out>.per_orient`): int32 header {n_orient, ntheta, n_alpha, n_beta},
+ * then n_orient FP64 weights, then n_orient x 16 x ntheta FP64 Mueller
+ * matrices without the 1/|k|^2 normalisation (the averaged matrix is
+ * sum_o w_o M_o / |k|^2).  Returns 0, or BEM_ERR_ARG when the file cannot be
+ * written. */
+int bem_write_per_orient(const char* path, int n_orient, int ntheta, int n_alpha, int n_beta, const double* weights,
+                         const double* M_orient);
+
 /* ---- mesh and basis ---------------------------------------------------- */
 /* Replaces the Mesh/RWG hand-off into assemble_pmchwt (pmchwt.h:12-19):
  * verts is nv x 3, tris is nt x 3 (0-based).  Builds the RWG basis in the
