@@ -83,6 +83,7 @@ def lib():
         "bem_factor": ([vp, i], i), "bem_set_Z": ([vp, _dp, i, i], i), "bem_get_pivots": ([vp, _ip], i),
         "bem_get_lu": ([vp, _dp], i), "bem_set_lu": ([vp, _dp, _ip, i], i),
         "bem_solve": ([vp, _dp, i, _dp, d, i, ctypes.POINTER(SolveStats)], i),
+        "bem_solve_gmres": ([vp, _dp, i, _dp, d, i, i, ctypes.POINTER(SolveStats)], i),
         "bem_rhs_planewave": ([vp, d, d, d, _dp, _dp, i, _dp], i),
         "bem_farfield": ([vp, _dp, _dp, _dp, d, d, d, i, i, i, _dp], i),
         "bem_orient_mueller": ([vp, d, d, d, _dp, _dp, i, _dp, i, i, _dp, _dp], i),
@@ -260,6 +261,29 @@ class Context:
         rc = self._L.bem_solve(self._h, bp, nrhs or 0, _p(X) if X is not None else None, ir_tol, ir_max,
                                ctypes.byref(st))
         self._check(rc, "solve", allow=(ERR_NOCONV,) if allow_noconv else ())
+        return (X.T.copy() if X is not None else None), st.as_dict()
+
+    def solve_gmres(self, B=None, tol=1e-10, restart=30, max_iter=200, return_X=True, n=None, nrhs=None,
+                    allow_noconv=False):
+        """Solve Z X = B by restarted flexible GMRES on the FP64 Z, preconditioned
+        by the context's factors (which may be those of an earlier matrix, see
+        bem_solve_gmres).  Arguments and results as for solve."""
+        st = SolveStats()
+        if B is not None:
+            B = np.asarray(B, dtype=np.complex128)
+            if B.ndim == 1:
+                B = B[:, None]
+            n, nrhs = B.shape
+            Bc = np.ascontiguousarray(B.T)
+            bp = _p(Bc)
+        else:
+            bp = None
+            if n is None:
+                n = 2 * self.N
+        X = np.zeros((nrhs, n), np.complex128) if (return_X and nrhs) else None
+        rc = self._L.bem_solve_gmres(self._h, bp, nrhs or 0, _p(X) if X is not None else None, tol, restart,
+                                     max_iter, ctypes.byref(st))
+        self._check(rc, "solve_gmres", allow=(ERR_NOCONV,) if allow_noconv else ())
         return (X.T.copy() if X is not None else None), st.as_dict()
 
     # -- excitation / far field --------------------------------------------

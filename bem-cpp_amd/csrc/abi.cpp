@@ -108,6 +108,7 @@ void free_matrices(bem_ctx* c) {
     c->inv32_bytes = c->inv64_bytes = 0;
     c->has_Z64 = c->has_Z32 = false;
     c->factored = -1;
+    c->factors_stale = false;
 }
 
 void free_rhs(bem_ctx* c) {
@@ -509,7 +510,13 @@ int bem_assemble(bem_ctx* c, double ker, double kei, double kir, double kii, dou
     BEM_HIP(c, hipGetLastError());
     c->has_Z64 = w64;
     c->has_Z32 = w32;
-    c->factored = -1;
+    // FP32 factors live in the FP32 working copy, which a Z32 assembly
+    // overwrites; other factors stay as the preconditioner of
+    // bem_solve_gmres but belong to an earlier matrix
+    if (c->factored == BEM_F32_3M && w32)
+        c->factored = -1;
+    else if (c->factored >= 0)
+        c->factors_stale = true;
     c->asm_params = p;
     c->asm_valid = true;
     return 0;
@@ -573,7 +580,7 @@ int bem_set_Z(bem_ctx* c, const double* Z, int n, int row_major) {
     BEM_HIP(c, hipMemcpy(c->Zi, im.data(), im.size() * sizeof(double), hipMemcpyHostToDevice));
     c->has_Z64 = true;
     c->has_Z32 = false;
-    c->factored = -1;
+    if (c->factored >= 0) c->factors_stale = true;  // kept for bem_solve_gmres
     c->asm_valid = false;
     return 0;
 }
@@ -603,6 +610,7 @@ int bem_factor(bem_ctx* c, int precision) {
     // the FP32 working copy now holds factors
     if (precision == BEM_F32_3M) c->has_Z32 = false;
     c->factored = info == 0 ? precision : -1;
+    c->factors_stale = false;
     return info;
 }
 
@@ -667,19 +675,17 @@ int bem_set_lu(bem_ctx* c, const double* LU, const int* ipiv, int n) {
     BEM_HIP(c, hipMemcpy(c->perm, perm.data(), ld * sizeof(int), hipMemcpyHostToDevice));
     if (int rc = lu_inverses_launch(c)) return rc;
     c->factored = BEM_F64;
+    c->factors_stale = false;
     c->has_Z64 = false;  // no system matrix: solves run without refinement
     return 0;
 }
 
-int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, int ir_max, bem_solve_stats* st) {
-    if (!c) return BEM_ERR_ARG;
-    KtScope kts{c};
-    if (c->factored < 0) return bem_fail(c, BEM_ERR_STATE, "solve: no factorisation");
-    if (nrhs < 1 && B) return bem_fail(c, BEM_ERR_ARG, "solve: nrhs must be positive");
-    if (ir_max < 0 || !(ir_tol >= 0.0)) return bem_fail(c, BEM_ERR_ARG, "solve: bad refinement parameters");
-    (void)hipSetDevice(c->device);
+// Right-hand sides from the host (column-major n x nrhs complex) into the
+// device planes, or the device's own (from bem_rhs_planewave) when B is null.
+static int load_rhs(bem_ctx* c, const double* B, int& nrhs, const char* who) {
     const int n = c->n;
     if (B) {
+        if (nrhs < 1) return bem_fail(c, BEM_ERR_ARG, std::string(who) + ": nrhs must be positive");
         if (int rc = ctx_ensure_rhs(c, nrhs)) return rc;
         const int ld = c->ldr;
         std::vector<double> re((size_t)c->npad * ld, 0.0), im((size_t)c->npad * ld, 0.0);
@@ -693,9 +699,36 @@ int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, i
         c->has_B = true;
         c->rhs_planewave = false;
     } else {
-        if (!c->has_B) return bem_fail(c, BEM_ERR_STATE, "solve: no right-hand sides on the device");
+        if (!c->has_B) return bem_fail(c, BEM_ERR_STATE, std::string(who) + ": no right-hand sides on the device");
         nrhs = c->nrhs;
     }
+    return 0;
+}
+
+// The device solutions into X (column-major n x nrhs complex).
+static int store_solution(bem_ctx* c, int nrhs, double* X) {
+    const int n = c->n, ld = c->ldr;
+    std::vector<double> re((size_t)n * ld), im((size_t)n * ld);
+    BEM_HIP(c, hipMemcpy(re.data(), c->Xr, re.size() * sizeof(double), hipMemcpyDeviceToHost));
+    BEM_HIP(c, hipMemcpy(im.data(), c->Xi, im.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < nrhs; j++)
+        for (int i = 0; i < n; i++) {
+            X[2 * ((size_t)j * n + i)] = re[(size_t)i * ld + j];
+            X[2 * ((size_t)j * n + i) + 1] = im[(size_t)i * ld + j];
+        }
+    return 0;
+}
+
+int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, int ir_max, bem_solve_stats* st) {
+    if (!c) return BEM_ERR_ARG;
+    KtScope kts{c};
+    if (c->factored < 0) return bem_fail(c, BEM_ERR_STATE, "solve: no factorisation");
+    if (c->factors_stale)
+        return bem_fail(c, BEM_ERR_STATE, "solve: the factors belong to an earlier matrix (factor again, or use "
+                                          "bem_solve_gmres)");
+    if (ir_max < 0 || !(ir_tol >= 0.0)) return bem_fail(c, BEM_ERR_ARG, "solve: bad refinement parameters");
+    (void)hipSetDevice(c->device);
+    if (int rc = load_rhs(c, B, nrhs, "solve")) return rc;
     int iters = 0;
     double r0 = 0, r1 = 0, ts = 0, ti = 0;
     const int rc = lu_solve_launch(c, nrhs, ir_tol, ir_max, &iters, &r0, &r1, &ts, &ti);
@@ -711,18 +744,39 @@ int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, i
         st->t_solve_ms = ts;
         st->t_ir_ms = ti;
     }
-    if (X) {
-        const int ld = c->ldr;
-        std::vector<double> re((size_t)n * ld), im((size_t)n * ld);
-        BEM_HIP(c, hipMemcpy(re.data(), c->Xr, re.size() * sizeof(double), hipMemcpyDeviceToHost));
-        BEM_HIP(c, hipMemcpy(im.data(), c->Xi, im.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int j = 0; j < nrhs; j++)
-            for (int i = 0; i < n; i++) {
-                X[2 * ((size_t)j * n + i)] = re[(size_t)i * ld + j];
-                X[2 * ((size_t)j * n + i) + 1] = im[(size_t)i * ld + j];
-            }
-    }
+    if (X)
+        if (int e = store_solution(c, nrhs, X)) return e;
     return rc == BEM_ERR_NOCONV ? bem_fail(c, rc, "solve: refinement did not reach the tolerance") : 0;
+}
+
+int bem_solve_gmres(bem_ctx* c, const double* B, int nrhs, double* X, double tol, int restart, int max_iter,
+                    bem_solve_stats* st) {
+    if (!c) return BEM_ERR_ARG;
+    KtScope kts{c};
+    if (c->factored < 0) return bem_fail(c, BEM_ERR_STATE, "solve_gmres: no factors for the preconditioner");
+    if (!c->has_Z64) return bem_fail(c, BEM_ERR_STATE, "solve_gmres: no FP64 matrix on the device");
+    if (!(tol > 0.0) || restart < 1 || restart > 200 || max_iter < 1)
+        return bem_fail(c, BEM_ERR_ARG, "solve_gmres: bad tolerance, restart length or iteration limit");
+    (void)hipSetDevice(c->device);
+    if (int rc = load_rhs(c, B, nrhs, "solve_gmres")) return rc;
+    int iters = 0;
+    double r0 = 0, r1 = 0, tms = 0;
+    const int rc = gmres_solve_launch(c, nrhs, tol, restart, max_iter, &iters, &r0, &r1, &tms);
+    if (rc < 0 && rc != BEM_ERR_NOCONV) return rc;
+    c->t_ms[2] = tms;
+    c->t_ms[3] = 0.0;
+    c->has_X = true;
+    if (st) {
+        st->iterations = iters;
+        st->residual0 = r0;
+        st->residual = r1;
+        st->converged = rc == 0 ? 1 : 0;
+        st->t_solve_ms = tms;
+        st->t_ir_ms = 0.0;
+    }
+    if (X)
+        if (int e = store_solution(c, nrhs, X)) return e;
+    return rc == BEM_ERR_NOCONV ? bem_fail(c, rc, "solve_gmres: no convergence within the iteration limit") : 0;
 }
 
 int bem_rhs_planewave(bem_ctx* c, double kr, double ki, double eta, const double* khat, const double* E0, int nrhs,
@@ -812,8 +866,9 @@ int bem_orient_mueller(bem_ctx* c, double kr, double ki, double eta, const doubl
 
 int bem_mueller_grad(bem_ctx* c, const double* dL_dM, double* grad) {
     if (!c || !dL_dM || !grad) return BEM_ERR_ARG;
-    if (c->last_norient == 0 || !c->asm_valid || c->factored < 0)
-        return bem_fail(c, BEM_ERR_STATE, "mueller_grad: needs assemble, factor, solve and orient_mueller first");
+    if (c->last_norient == 0 || !c->asm_valid || c->factored < 0 || c->factors_stale)
+        return bem_fail(c, BEM_ERR_STATE,
+                        "mueller_grad: needs assemble, factor (of this matrix), solve and orient_mueller first");
     (void)hipSetDevice(c->device);
     EventTimer tm(c);
     const int rc = adjoint_launch(c, dL_dM, grad);

@@ -142,6 +142,7 @@ struct bem_ctx {
     int* ipiv = nullptr;   // [npad] LAPACK-style swaps (0-based row swapped with row i)
     int* perm = nullptr;   // [npad] row permutation: row i of PZ is row perm[i] of Z
     int factored = -1;     // -1 none, BEM_F64, BEM_F32_3M
+    bool factors_stale = false;  // the factors belong to an earlier matrix (preconditioner only)
     AsmParams asm_params{};
     bool asm_valid = false;
 
@@ -191,6 +192,16 @@ int ctx_scratch(bem_ctx* c, size_t bytes, void** out);
 int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32);
 int lu_factor_launch(bem_ctx* c, int precision);
 int lu_inverses_launch(bem_ctx* c);  // FP64 diagonal-block inverses of factors in Vr/Vi
+// D = (P^T L U)^{-1} S with the context's factors (FP32 or FP64), S and D
+// planar FP64 npad x ldr (D must not alias S); work holds
+// lu_precond_work_bytes(c) bytes.  Enqueued on the context stream.
+size_t lu_precond_work_bytes(const bem_ctx* c);
+int lu_precond_apply(bem_ctx* c, const double* Sr, const double* Si, double* Dr, double* Di, void* work);
+// Restarted flexible GMRES for Z X = B (Z the FP64 matrix on the device, B
+// and X the context's right-hand sides and solutions) with the factors as
+// right preconditioner (gmres.hip).
+int gmres_solve_launch(bem_ctx* c, int nrhs, double tol, int restart, int max_iter, int* iters, double* res0,
+                       double* res, double* t_ms);
 int lu_solve_launch(bem_ctx* c, int nrhs, double ir_tol, int ir_max, int* iters, double* res0, double* res,
                     double* t_solve_ms, double* t_ir_ms);
 int rhs_launch(bem_ctx* c, double kr, double ki, double eta, const double* khat, const double* E0, int nrhs);

@@ -109,6 +109,41 @@ double max_ratio(const std::vector<double>& r2, const std::vector<double>& b2, i
 
 }  // namespace
 
+size_t lu_precond_work_bytes(const bem_ctx* c) {
+    const size_t ysz = c->factored == BEM_F32_3M ? sizeof(float) : sizeof(double);
+    return 4 * (size_t)c->npad * c->ldr * ysz;
+}
+
+int lu_precond_apply(bem_ctx* c, const double* Sr, const double* Si, double* Dr, double* Di, void* work) {
+    const int n = c->npad, ncol = c->ldr;
+    const long cnt = (long)n * ncol;
+    hipStream_t s = c->stream;
+    const dim3 g1((unsigned)((cnt + 255) / 256));
+    if (c->factored == BEM_F32_3M) {
+        float* Yr = static_cast<float*>(work);
+        float* Yi = Yr + cnt;
+        float* Qr = Yi + cnt;
+        float* Qi = Qr + cnt;
+        hipLaunchKernelGGL(k_gather_rows<float>, g1, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ncol, Yr, Yi,
+                           (long)ncol);
+        lu_apply<float>(c, c->Wr, c->Wi, c->inv32, Yr, Yi, Qr, Qi, ncol, ncol);
+        hipLaunchKernelGGL(k_accumulate<float>, g1, dim3(256), 0, s, cnt, Yr, Yi, Dr, Di, 1);
+    } else if (c->factored == BEM_F64) {
+        double* Yr = static_cast<double*>(work);
+        double* Yi = Yr + cnt;
+        double* Qr = Yi + cnt;
+        double* Qi = Qr + cnt;
+        hipLaunchKernelGGL(k_gather_rows<double>, g1, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ncol, Yr, Yi,
+                           (long)ncol);
+        lu_apply<double>(c, c->Vr, c->Vi, c->inv64, Yr, Yi, Qr, Qi, ncol, ncol);
+        hipLaunchKernelGGL(k_accumulate<double>, g1, dim3(256), 0, s, cnt, Yr, Yi, Dr, Di, 1);
+    } else {
+        return bem_fail(c, BEM_ERR_STATE, "preconditioner: no factorisation");
+    }
+    BEM_HIP(c, hipGetLastError());
+    return 0;
+}
+
 int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, double* res0, double* res,
                     double* t_solve_ms, double* t_ir_ms) {
     const int n = c->npad, ldr = c->ldr, ncol = ldr;

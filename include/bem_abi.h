@@ -192,6 +192,22 @@ int bem_set_lu(bem_ctx* ctx, const double* LU, const int* ipiv, int n);
  * solution is refined against the FP64 Z until max_j ||b_j - Z x_j||/||b_j||
  * <= ir_tol or ir_max sweeps (ir_max = 0: no refinement).  stats may be NULL. */
 int bem_solve(bem_ctx* ctx, const double* B, int nrhs, double* X, double ir_tol, int ir_max, bem_solve_stats* stats);
+/* Replaces gmres_solve_paired (block_gmres.h:52-61) on the dense operator and
+ * solve_gmres_aext (python/bem_core.py:1104-1123): restarted flexible GMRES
+ * on the FP64 Z of the device for all right-hand sides at once (x_0 = 0),
+ * preconditioned on the right by the context's factors, until
+ * max_j ||b_j - Z x_j||/||b_j|| <= tol or max_iter iterations (restart
+ * length `restart`, 1..200).  The factors may belong to an earlier matrix:
+ * bem_assemble without Z32, or bem_set_Z, keeps them for this call (and
+ * bem_solve then refuses them).  With the factors of Z at m = 1 this is the
+ * reference's A_ext preconditioner (2 A_ext when eta_ext = 1), reused over
+ * refractive indices.  B, X, nrhs as for bem_solve; stats->iterations counts
+ * the GMRES iterations (one matvec and one preconditioner solve each),
+ * residual0 = 1, residual = the final true relative residual.  Returns 0,
+ * BEM_ERR_NOCONV when the limit is reached first (X holds the last iterate),
+ * or < 0. */
+int bem_solve_gmres(bem_ctx* ctx, const double* B, int nrhs, double* X, double tol, int restart, int max_iter,
+                    bem_solve_stats* stats);
 
 /* ---- excitation -------------------------------------------------------- */
 /* Replaces compute_rhs_planewave (rhs.h:11-16, rhs.cpp:6-64), batched: RHS j
