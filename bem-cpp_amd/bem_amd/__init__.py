@@ -83,7 +83,10 @@ def lib():
         "bem_factor": ([vp, i], i), "bem_set_Z": ([vp, _dp, i, i], i), "bem_get_pivots": ([vp, _ip], i),
         "bem_get_lu": ([vp, _dp], i), "bem_set_lu": ([vp, _dp, _ip, i], i),
         "bem_solve": ([vp, _dp, i, _dp, d, i, ctypes.POINTER(SolveStats)], i),
-        "bem_solve_gmres": ([vp, _dp, i, _dp, d, i, i, ctypes.POINTER(SolveStats)], i),
+        "bem_solve_gmres": ([vp, _dp, i, _dp, d, i, i, i, ctypes.POINTER(SolveStats)], i),
+        "bem_block_jacobi": ([vp, d, i, i, _dp], i),
+        "bem_block_partition": ([_dp, i, _ip, i, i, d, i, i, _ip, _ip, _ip], i),
+        "bem_precond_apply": ([vp, i, _dp, i, _dp], i),
         "bem_rhs_planewave": ([vp, d, d, d, _dp, _dp, i, _dp], i),
         "bem_farfield": ([vp, _dp, _dp, _dp, d, d, d, i, i, i, _dp], i),
         "bem_orient_mueller": ([vp, d, d, d, _dp, _dp, i, _dp, i, i, _dp, _dp], i),
@@ -149,6 +152,46 @@ def write_per_orient(path, weights, M_orient, n_alpha, n_beta):
                                     _p(Mo))
     if rc:
         raise BemError(rc, f"bem_write_per_orient({path})")
+
+
+PRECONDS = {"lu": 0, "block": 1, "none": 2}
+
+
+def block_partition(verts, tris, quad_order=7, radius_mult=2.0, max_block_rwg=1500, overlap_layers=1):
+    """The block-Jacobi / RAS partition of the RWGs (precond.cu:414-605) on the
+    host: a list of (own, extended) index arrays (RWG order of Context.rwg)."""
+    v = _cd(verts).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+    n_ext = ctypes.c_int()
+    nb = lib().bem_block_partition(_p(v), len(v), _pi(t), len(t), int(quad_order), float(radius_mult),
+                                   int(max_block_rwg), int(overlap_layers), ctypes.byref(n_ext), None, None)
+    if nb < 0:
+        raise BemError(nb, "bem_block_partition")
+    sizes = np.zeros(2 * nb, np.int32)
+    rwg = np.zeros(max(n_ext.value, 1), np.int32)
+    lib().bem_block_partition(_p(v), len(v), _pi(t), len(t), int(quad_order), float(radius_mult), int(max_block_rwg),
+                              int(overlap_layers), ctypes.byref(n_ext), _pi(sizes), _pi(rwg))
+    out, o = [], 0
+    for b in range(nb):
+        own, ext = int(sizes[2 * b]), int(sizes[2 * b + 1])
+        out.append((rwg[o:o + own].copy(), rwg[o:o + ext].copy()))
+        o += ext
+    return out
+
+
+def auto_precond_params(N, ka):
+    """The reference's automatic choice (AutoPrecondParams::compute,
+    precond.cu:896-970, without its GPU-memory cap): N <= 500 -> the 2 x 2
+    self-interaction blocks, else RAS with one overlap layer and
+    block size clamp(sqrt(3e8 / N), 100, 600) (at most 200 for ka > 15 and
+    N > 10000).  Returns dict(radius_mult, max_block_rwg, overlap_layers)."""
+    if N <= 500:
+        return dict(radius_mult=2.0, max_block_rwg=1, overlap_layers=0)
+    bs = int(np.sqrt(3e8 / N))
+    bs = max(min(bs, 600), 100)
+    if ka > 15 and N > 10000:
+        bs = min(bs, 200)
+    return dict(radius_mult=2.0, max_block_rwg=bs, overlap_layers=1)
 
 
 class Context:
@@ -264,10 +307,12 @@ class Context:
         return (X.T.copy() if X is not None else None), st.as_dict()
 
     def solve_gmres(self, B=None, tol=1e-10, restart=30, max_iter=200, return_X=True, n=None, nrhs=None,
-                    allow_noconv=False):
+                    allow_noconv=False, precond="lu"):
         """Solve Z X = B by restarted flexible GMRES on the FP64 Z, preconditioned
-        by the context's factors (which may be those of an earlier matrix, see
-        bem_solve_gmres).  Arguments and results as for solve."""
+        by the context's factors (precond="lu"; they may be those of an earlier
+        matrix, see bem_solve_gmres), the block-Jacobi preconditioner
+        ("block", see block_jacobi) or nothing ("none").  Arguments and results
+        as for solve."""
         st = SolveStats()
         if B is not None:
             B = np.asarray(B, dtype=np.complex128)
@@ -282,9 +327,29 @@ class Context:
                 n = 2 * self.N
         X = np.zeros((nrhs, n), np.complex128) if (return_X and nrhs) else None
         rc = self._L.bem_solve_gmres(self._h, bp, nrhs or 0, _p(X) if X is not None else None, tol, restart,
-                                     max_iter, ctypes.byref(st))
+                                     max_iter, PRECONDS[precond], ctypes.byref(st))
         self._check(rc, "solve_gmres", allow=(ERR_NOCONV,) if allow_noconv else ())
         return (X.T.copy() if X is not None else None), st.as_dict()
+
+    def block_jacobi(self, radius_mult=2.0, max_block_rwg=1500, overlap_layers=1):
+        """Builds the block-Jacobi / RAS preconditioner from the device's FP64 Z
+        (precond.cu:407-895); returns (block count, build ms)."""
+        t = ctypes.c_double()
+        nb = self._check(self._L.bem_block_jacobi(self._h, float(radius_mult), int(max_block_rwg),
+                                                  int(overlap_layers), ctypes.byref(t)), "block_jacobi")
+        return nb, t.value
+
+    def precond_apply(self, R, precond="block"):
+        """M^{-1} R for R (n, nrhs) or (n,)."""
+        R = np.asarray(R, dtype=np.complex128)
+        one = R.ndim == 1
+        if one:
+            R = R[:, None]
+        n, nrhs = R.shape
+        Rc = np.ascontiguousarray(R.T)
+        Zo = np.zeros((nrhs, n), np.complex128)
+        self._check(self._L.bem_precond_apply(self._h, PRECONDS[precond], _p(Rc), nrhs, _p(Zo)), "precond_apply")
+        return Zo[0].copy() if one else Zo.T.copy()
 
     # -- excitation / far field --------------------------------------------
     def rhs_planewave(self, k_ext: complex, eta_ext: float, k_hat, E0, return_B=False):

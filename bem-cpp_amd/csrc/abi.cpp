@@ -91,6 +91,7 @@ void free_mesh(bem_ctx* c) {
     dfree(d.near);
     dfree(d.pv);
     d = DevMesh();
+    bj_free(c);  // the blocks belong to the mesh
 }
 
 void free_matrices(bem_ctx* c) {
@@ -750,10 +751,15 @@ int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, i
 }
 
 int bem_solve_gmres(bem_ctx* c, const double* B, int nrhs, double* X, double tol, int restart, int max_iter,
-                    bem_solve_stats* st) {
+                    int precond, bem_solve_stats* st) {
     if (!c) return BEM_ERR_ARG;
     KtScope kts{c};
-    if (c->factored < 0) return bem_fail(c, BEM_ERR_STATE, "solve_gmres: no factors for the preconditioner");
+    if (precond != BEM_PRECOND_LU && precond != BEM_PRECOND_BLOCK && precond != BEM_PRECOND_NONE)
+        return bem_fail(c, BEM_ERR_ARG, "solve_gmres: unknown preconditioner");
+    if (precond == BEM_PRECOND_LU && c->factored < 0)
+        return bem_fail(c, BEM_ERR_STATE, "solve_gmres: no factors for the preconditioner");
+    if (precond == BEM_PRECOND_BLOCK && (!c->bj.valid || 2 * c->bj.N != c->n))
+        return bem_fail(c, BEM_ERR_STATE, "solve_gmres: no block-Jacobi preconditioner for this system");
     if (!c->has_Z64) return bem_fail(c, BEM_ERR_STATE, "solve_gmres: no FP64 matrix on the device");
     if (!(tol > 0.0) || restart < 1 || restart > 200 || max_iter < 1)
         return bem_fail(c, BEM_ERR_ARG, "solve_gmres: bad tolerance, restart length or iteration limit");
@@ -761,7 +767,7 @@ int bem_solve_gmres(bem_ctx* c, const double* B, int nrhs, double* X, double tol
     if (int rc = load_rhs(c, B, nrhs, "solve_gmres")) return rc;
     int iters = 0;
     double r0 = 0, r1 = 0, tms = 0;
-    const int rc = gmres_solve_launch(c, nrhs, tol, restart, max_iter, &iters, &r0, &r1, &tms);
+    const int rc = gmres_solve_launch(c, nrhs, tol, restart, max_iter, precond, &iters, &r0, &r1, &tms);
     if (rc < 0 && rc != BEM_ERR_NOCONV) return rc;
     c->t_ms[2] = tms;
     c->t_ms[3] = 0.0;
@@ -777,6 +783,73 @@ int bem_solve_gmres(bem_ctx* c, const double* B, int nrhs, double* X, double tol
     if (X)
         if (int e = store_solution(c, nrhs, X)) return e;
     return rc == BEM_ERR_NOCONV ? bem_fail(c, rc, "solve_gmres: no convergence within the iteration limit") : 0;
+}
+
+int bem_block_jacobi(bem_ctx* c, double radius_mult, int max_block_rwg, int overlap_layers, double* build_ms) {
+    if (!c) return BEM_ERR_ARG;
+    if (!(radius_mult > 0.0) || max_block_rwg < 1 || overlap_layers < 0)
+        return bem_fail(c, BEM_ERR_ARG, "block_jacobi: bad parameters");
+    if (!c->has_mesh || !c->has_Z64 || c->n != 2 * c->rwg.N)
+        return bem_fail(c, BEM_ERR_STATE, "block_jacobi: needs the mesh and its FP64 Z on the device");
+    (void)hipSetDevice(c->device);
+    double t = 0.0;
+    const int nb = bj_build_launch(c, radius_mult, max_block_rwg, overlap_layers, &t);
+    if (nb < 0) return nb;
+    if (build_ms) *build_ms = t;
+    return nb;
+}
+
+int bem_block_partition(const double* verts, int nv, const int* tris, int nt, int quad_order, double radius_mult,
+                        int max_block_rwg, int overlap_layers, int* n_ext, int* sizes, int* rwg_ext) {
+    if (!verts || !tris || nv < 3 || nt < 1 || !n_ext || !(radius_mult > 0.0) || max_block_rwg < 1 ||
+        overlap_layers < 0)
+        return BEM_ERR_ARG;
+    Mesh m;
+    m.verts.resize(nv);
+    for (int i = 0; i < nv; i++) m.verts[i] = Vec3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    m.tris.assign(tris, tris + 3 * (size_t)nt);
+    for (int v : m.tris)
+        if (v < 0 || v >= nv) return BEM_ERR_MESH;
+    const RWG r = build_rwg(m);
+    if (r.N == 0) return BEM_ERR_MESH;
+    const BlockPartition P = block_partition(m, r, tri_quadrature(quad_order), radius_mult, max_block_rwg,
+                                             overlap_layers);
+    int total = 0;
+    for (size_t b = 0; b < P.own.size(); b++) {
+        if (sizes) {
+            sizes[2 * b] = (int)P.own[b].size();
+            sizes[2 * b + 1] = (int)P.ext[b].size();
+        }
+        if (rwg_ext)
+            for (int x : P.ext[b]) rwg_ext[total++] = x;
+        else
+            total += (int)P.ext[b].size();
+    }
+    *n_ext = total;
+    return (int)P.own.size();
+}
+
+int bem_precond_apply(bem_ctx* c, int precond, const double* R, int nrhs, double* Zout) {
+    if (!c || !R || !Zout || nrhs < 1) return BEM_ERR_ARG;
+    KtScope kts{c};
+    if (precond == BEM_PRECOND_LU && c->factored < 0)
+        return bem_fail(c, BEM_ERR_STATE, "precond_apply: no factors");
+    if (precond == BEM_PRECOND_BLOCK && (!c->bj.valid || 2 * c->bj.N != c->n))
+        return bem_fail(c, BEM_ERR_STATE, "precond_apply: no block-Jacobi preconditioner for this system");
+    if (precond != BEM_PRECOND_LU && precond != BEM_PRECOND_BLOCK)
+        return bem_fail(c, BEM_ERR_ARG, "precond_apply: unknown preconditioner");
+    (void)hipSetDevice(c->device);
+    // R goes through the right-hand-side planes, the result through the solution planes
+    if (int rc = load_rhs(c, R, nrhs, "precond_apply")) return rc;
+    const size_t work = precond == BEM_PRECOND_LU ? lu_precond_work_bytes(c) : bj_work_bytes(c);
+    void* scr = nullptr;
+    if (int rc = ctx_scratch(c, work + 256, &scr)) return rc;
+    const int rc = precond == BEM_PRECOND_LU ? lu_precond_apply(c, c->Br, c->Bi, c->Xr, c->Xi, scr)
+                                             : bj_apply(c, c->Br, c->Bi, c->Xr, c->Xi, scr);
+    if (rc) return rc;
+    BEM_HIP(c, hipStreamSynchronize(c->stream));
+    c->has_X = true;
+    return store_solution(c, nrhs, Zout);
 }
 
 int bem_rhs_planewave(bem_ctx* c, double kr, double ki, double eta, const double* khat, const double* E0, int nrhs,

@@ -103,6 +103,21 @@ struct GemmPlan {
     GemmPlan(int l, int s) : large(l), skinny(s) {}
 };
 
+// Block-Jacobi / restricted additive Schwarz preconditioner on the device
+// (precond.hip): per block the extended RWG list, its row-major factors and
+// pivots.
+struct BlockJacobi {
+    bool valid = false;
+    int nb = 0, n2max = 0, N = 0;
+    long work_rows = 0;  // sum of the blocks' 2 Be
+    size_t lu_bytes = 0;
+    int woff_ncol = -1;  // column count the device work offsets were made for
+    int *rwg = nullptr, *B = nullptr, *Be = nullptr, *roff = nullptr, *poff = nullptr, *piv = nullptr;
+    long *loff = nullptr, *woff = nullptr;
+    double *Lr = nullptr, *Li = nullptr;
+    std::vector<std::vector<int>> own, ext;
+};
+
 struct bem_ctx {
     int device = 0;
     GemmPlan gemm_plan[2] = {GemmPlan(0, 2), GemmPlan(1, 2)};  // [0] FP32 3M, [1] FP64 (defaults before tuning)
@@ -141,6 +156,7 @@ struct bem_ctx {
     size_t inv64_bytes = 0;
     int* ipiv = nullptr;   // [npad] LAPACK-style swaps (0-based row swapped with row i)
     int* perm = nullptr;   // [npad] row permutation: row i of PZ is row perm[i] of Z
+    BlockJacobi bj;        // block-Jacobi preconditioner (bem_block_jacobi)
     int factored = -1;     // -1 none, BEM_F64, BEM_F32_3M
     bool factors_stale = false;  // the factors belong to an earlier matrix (preconditioner only)
     AsmParams asm_params{};
@@ -197,11 +213,18 @@ int lu_inverses_launch(bem_ctx* c);  // FP64 diagonal-block inverses of factors 
 // lu_precond_work_bytes(c) bytes.  Enqueued on the context stream.
 size_t lu_precond_work_bytes(const bem_ctx* c);
 int lu_precond_apply(bem_ctx* c, const double* Sr, const double* Si, double* Dr, double* Di, void* work);
+// Block-Jacobi preconditioner (precond.hip): build from the FP64 Z and the
+// mesh (returns the block count or < 0), free, apply to planar FP64 npad x ldr
+// vectors (work: bj_work_bytes(c) bytes).
+int bj_build_launch(bem_ctx* c, double radius_mult, int max_block_rwg, int overlap_layers, double* t_ms);
+void bj_free(bem_ctx* c);
+size_t bj_work_bytes(const bem_ctx* c);
+int bj_apply(bem_ctx* c, const double* Rr, const double* Ri, double* Zr, double* Zi, void* work);
 // Restarted flexible GMRES for Z X = B (Z the FP64 matrix on the device, B
 // and X the context's right-hand sides and solutions) with the factors as
 // right preconditioner (gmres.hip).
-int gmres_solve_launch(bem_ctx* c, int nrhs, double tol, int restart, int max_iter, int* iters, double* res0,
-                       double* res, double* t_ms);
+int gmres_solve_launch(bem_ctx* c, int nrhs, double tol, int restart, int max_iter, int precond, int* iters,
+                       double* res0, double* res, double* t_ms);
 int lu_solve_launch(bem_ctx* c, int nrhs, double ir_tol, int ir_max, int* iters, double* res0, double* res,
                     double* t_solve_ms, double* t_ir_ms);
 int rhs_launch(bem_ctx* c, double kr, double ki, double eta, const double* khat, const double* E0, int nrhs);

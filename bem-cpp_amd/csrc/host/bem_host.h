@@ -102,6 +102,15 @@ struct TriQuad {
 };
 TriQuad tri_quadrature(int order);
 
+// ---- block partition (block-Jacobi / RAS preconditioner) -------------------
+struct BlockPartition {
+    std::vector<std::vector<int>> own;  // RWG indices of each block, sorted
+    std::vector<std::vector<int>> ext;  // own first, then the overlap RWGs, sorted
+    double cell_size = 0.0, avg_extent = 0.0;
+};
+BlockPartition block_partition(const Mesh& mesh, const RWG& rwg, const TriQuad& quad, double radius_mult,
+                               int max_block_rwg, int overlap_layers);
+
 // ---- orientations ----------------------------------------------------------
 struct Mat3 {
     double m[3][3];

@@ -9,7 +9,8 @@
 // right-hand side runs its own Krylov process in lock step with the others:
 // one matvec per iteration is one FP64 complex GEMM Z V (n x n x nrhs, 3M
 // form, split K), one preconditioner application is one multi-RHS solve with
-// the factors on the device (FP32 factors on the matrix cores, or FP64).
+// the factors on the device (FP32 factors on the matrix cores, or FP64), or
+// the block-Jacobi / RAS preconditioner of precond.hip, or none.
 // The factors may belong to an earlier matrix: with the factors of Z at
 // m = 1 (which is 2 A_ext when eta_ext = 1) this is the reference's A_ext
 // preconditioner, reused across refractive indices; with the factors of Z
@@ -133,8 +134,8 @@ typedef std::complex<double> cd;
 
 }  // namespace
 
-int gmres_solve_launch(bem_ctx* c, int nrhs, double tol, int restart, int max_iter, int* iters, double* res0,
-                       double* res, double* t_ms) {
+int gmres_solve_launch(bem_ctx* c, int nrhs, double tol, int restart, int max_iter, int precond, int* iters,
+                       double* res0, double* res, double* t_ms) {
     const auto t0 = std::chrono::steady_clock::now();
     const int n = c->npad, ncol = c->ldr, nphys = c->n;
     const long cnt = (long)n * ncol;
@@ -146,7 +147,10 @@ int gmres_solve_launch(bem_ctx* c, int nrhs, double tol, int restart, int max_it
     const size_t vec = 2 * (size_t)cnt;  // doubles per planar block vector
     const size_t nP = 2 * (size_t)(m + 1) * nchunk * ncol, nH = 2 * (size_t)(m + 1) * ncol;
     const size_t doubles = vec * (2 * (size_t)m + 3) + nP + 2 * nH + 2 * (size_t)ncol;
-    const size_t bytes = doubles * sizeof(double) + lu_precond_work_bytes(c) + 256;
+    const size_t pwork = precond == BEM_PRECOND_LU ? lu_precond_work_bytes(c)
+                         : precond == BEM_PRECOND_BLOCK ? bj_work_bytes(c)
+                                                        : 0;
+    const size_t bytes = doubles * sizeof(double) + pwork + 256;
     void* scr = nullptr;
     if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
     double* base = static_cast<double*>(scr);
@@ -245,7 +249,14 @@ int gmres_solve_launch(bem_ctx* c, int nrhs, double tol, int restart, int max_it
         if (int rc = scale_into(r, Vk(0))) return rc;
         int it = 0;
         for (; it < m && total < max_iter; it++, total++) {
-            if (int rc = lu_precond_apply(c, re(Vk(it)), im(Vk(it)), re(Zk(it)), im(Zk(it)), work)) return rc;
+            int prc = 0;
+            if (precond == BEM_PRECOND_LU)
+                prc = lu_precond_apply(c, re(Vk(it)), im(Vk(it)), re(Zk(it)), im(Zk(it)), work);
+            else if (precond == BEM_PRECOND_BLOCK)
+                prc = bj_apply(c, re(Vk(it)), im(Vk(it)), re(Zk(it)), im(Zk(it)), work);
+            else
+                BEM_HIP(c, hipMemcpyAsync(Zk(it), Vk(it), vec * sizeof(double), hipMemcpyDeviceToDevice, s));
+            if (prc) return prc;
             if (int rc = cgemm_splitk_ctx<double>(c, n, ncol, n, c->Zr, c->Zi, n, re(Zk(it)), im(Zk(it)), ncol, re(w),
                                                   im(w), ncol, 1, 2))
                 return rc;

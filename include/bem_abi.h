@@ -43,6 +43,13 @@ enum {
     BEM_F32_3M = 1   /* complex FP32 blocked LU, trailing updates as 3 real MFMA GEMMs */
 };
 
+/* Preconditioners of bem_solve_gmres and bem_precond_apply. */
+enum {
+    BEM_PRECOND_LU = 0,    /* the context's LU factors (bem_factor), possibly of an earlier matrix */
+    BEM_PRECOND_BLOCK = 1, /* block-Jacobi / restricted additive Schwarz (bem_block_jacobi) */
+    BEM_PRECOND_NONE = 2   /* none */
+};
+
 /* Assembly flags (bem_assemble). */
 enum {
     BEM_ASM_Z64 = 1u, /* keep the FP64 copy of Z (needed by get_Z, the FP64 LU and refinement) */
@@ -195,7 +202,8 @@ int bem_solve(bem_ctx* ctx, const double* B, int nrhs, double* X, double ir_tol,
 /* Replaces gmres_solve_paired (block_gmres.h:52-61) on the dense operator and
  * solve_gmres_aext (python/bem_core.py:1104-1123): restarted flexible GMRES
  * on the FP64 Z of the device for all right-hand sides at once (x_0 = 0),
- * preconditioned on the right by the context's factors, until
+ * preconditioned on the right by `precond` (BEM_PRECOND_LU: the context's
+ * factors; BEM_PRECOND_BLOCK: bem_block_jacobi; BEM_PRECOND_NONE), until
  * max_j ||b_j - Z x_j||/||b_j|| <= tol or max_iter iterations (restart
  * length `restart`, 1..200).  The factors may belong to an earlier matrix:
  * bem_assemble without Z32, or bem_set_Z, keeps them for this call (and
@@ -207,7 +215,27 @@ int bem_solve(bem_ctx* ctx, const double* B, int nrhs, double* X, double ir_tol,
  * BEM_ERR_NOCONV when the limit is reached first (X holds the last iterate),
  * or < 0. */
 int bem_solve_gmres(bem_ctx* ctx, const double* B, int nrhs, double* X, double tol, int restart, int max_iter,
-                    bem_solve_stats* stats);
+                    int precond, bem_solve_stats* stats);
+/* Replaces the block-Jacobi / RAS build of NearFieldPrecond (precond.cu:407-895;
+ * precond.h:38-50) on the dense path: the RWGs are split into spatial-cell
+ * blocks (cell side = bounding box / (2.5 / radius_mult), blocks above
+ * max_block_rwg bisected; max_block_rwg = 1 gives one block per RWG, the 2 x 2
+ * self-interaction preconditioner of precond.cu:976-1090), each extended by
+ * the RWGs within overlap_layers * 2 mean element extents (RAS), and each
+ * extended block's 2B x 2B sub-matrix of the device's FP64 Z is factored on
+ * the device.  Needs the mesh and the FP64 Z; build_ms (nullable) receives the
+ * build time.  Returns the block count or < 0. */
+int bem_block_jacobi(bem_ctx* ctx, double radius_mult, int max_block_rwg, int overlap_layers, double* build_ms);
+/* The same partition on the host without a context (precond.cu:414-605):
+ * returns the block count; *n_ext receives the total extended size; sizes
+ * (nullable, 2 x count ints) receives own and extended sizes per block,
+ * rwg_ext (nullable, *n_ext ints) the extended RWG lists (own first), in the
+ * RWG order of bem_get_rwg. */
+int bem_block_partition(const double* verts, int nv, const int* tris, int nt, int quad_order, double radius_mult,
+                        int max_block_rwg, int overlap_layers, int* n_ext, int* sizes, int* rwg_ext);
+/* Applies a preconditioner (BEM_PRECOND_*) to R (n x nrhs column-major
+ * complex): Zout = M^{-1} R (the apply of precond.cu:1154-1265, 1529-1550). */
+int bem_precond_apply(bem_ctx* ctx, int precond, const double* R, int nrhs, double* Zout);
 
 /* ---- excitation -------------------------------------------------------- */
 /* Replaces compute_rhs_planewave (rhs.h:11-16, rhs.cpp:6-64), batched: RHS j
