@@ -104,17 +104,14 @@ struct GemmPlan {
 };
 
 // Block-Jacobi / restricted additive Schwarz preconditioner on the device
-// (precond.hip): per block the extended RWG list, its row-major factors and
-// pivots.
+// (precond.hip): per block the extended RWG list and the inverse of its
+// 2Be x 2Be sub-matrix of Z in a zero-padded ld x ld slot.
 struct BlockJacobi {
     bool valid = false;
-    int nb = 0, n2max = 0, N = 0;
-    long work_rows = 0;  // sum of the blocks' 2 Be
-    size_t lu_bytes = 0;
-    int woff_ncol = -1;  // column count the device work offsets were made for
+    int nb = 0, n2max = 0, ld = 0, N = 0;
+    size_t bytes = 0;
     int *rwg = nullptr, *B = nullptr, *Be = nullptr, *roff = nullptr, *poff = nullptr, *piv = nullptr;
-    long *loff = nullptr, *woff = nullptr;
-    double *Lr = nullptr, *Li = nullptr;
+    double *Ar = nullptr, *Ai = nullptr;
     std::vector<std::vector<int>> own, ext;
 };
 

@@ -1,6 +1,7 @@
 """Block-Jacobi / RAS preconditioner on the GPU (precond.cu:407-895 build,
-1303-1410 apply): the device apply against numpy (each extended block's
-sub-matrix of Z solved by LAPACK, own unknowns kept), the one-RWG blocks
+1303-1410 apply; here block inverses applied by one batched GEMM): the
+device apply against numpy (each extended block's sub-matrix of Z solved by
+LAPACK, own unknowns kept), the one-RWG blocks
 (the 2 x 2 self-interaction preconditioner), GMRES with it against a direct
 solve, and an exactly block-diagonal matrix where it is the inverse."""
 import numpy as np
