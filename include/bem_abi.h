@@ -233,8 +233,10 @@ int bem_block_jacobi(bem_ctx* ctx, double radius_mult, int max_block_rwg, int ov
  * RWG order of bem_get_rwg. */
 int bem_block_partition(const double* verts, int nv, const int* tris, int nt, int quad_order, double radius_mult,
                         int max_block_rwg, int overlap_layers, int* n_ext, int* sizes, int* rwg_ext);
-/* Applies a preconditioner (BEM_PRECOND_*) to R (n x nrhs column-major
- * complex): Zout = M^{-1} R (the apply of precond.cu:1154-1265, 1529-1550). */
+/* Applies a preconditioner (BEM_PRECOND_LU or BEM_PRECOND_BLOCK) to R
+ * (n x nrhs column-major complex): Zout = M^{-1} R (the apply of
+ * precond.cu:1154-1265, 1529-1550).  Like bem_solve with B given, it replaces
+ * the device right-hand sides (with R) and solutions (with Zout). */
 int bem_precond_apply(bem_ctx* ctx, int precond, const double* R, int nrhs, double* Zout);
 
 /* ---- excitation -------------------------------------------------------- */
