@@ -233,6 +233,19 @@ def main():
         log(f"instrumented step: {ph_instr}")
     ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
 
+    # adjoint gradient of the averaged-M11 loss L = sum_t w_t log M11(theta_t)
+    # (w uniform in cos theta) with respect to (Re m, Im m, ka), through the
+    # FP32 factorisation of the last step (one adjoint solve + dZ/dp contraction)
+    wt = np.sin(theta) * np.pi / (len(theta) - 1)
+    G = np.zeros((4, 4, len(theta)))
+    G[0, 0] = wt / M32[0, 0]
+    t_g = time.perf_counter()
+    grad = ctx.mueller_grad(G)
+    adjoint = dict(loss="sum_t w_t log M11(theta_t), w uniform in cos theta",
+                   grad_re_m_im_m_ka=[float(x) for x in grad], device_ms=float(ctx.timings()["adjoint_ms"]),
+                   wall_ms=(time.perf_counter() - t_g) * 1e3)
+    log(f"adjoint gradient: {adjoint}")
+
     # FP64 LU on the same GPU: the comparison and the FP64 reference for the Mueller error
     plan32 = ctx.gemm_plan_info("fp32")
     fp64 = {}
@@ -298,7 +311,8 @@ def main():
         "config": {
             "workload": (f"{shape} {'AR ' + str(param) if shape == 'hex' else 'r=1'} ref {ref}: T={len(tris)}, "
                          f"N={N}, 2N={n}; ka={ka}, m={m.real}{m.imag:+}i; 7-point quadrature; "
-                         f"{n_orient} orientations x 2 polarisations = {nrhs} RHS; {ntheta} angles"),
+                         f"{n_orient} orientations x 2 polarisations = {nrhs} RHS; {ntheta} angles; "
+                         "adjoint gradient of the averaged-M11 loss reported beside the timed step"),
             "n": n, "nrhs": nrhs, "n_orient": n_orient, "ntheta": ntheta,
         },
         "z_entries_per_s": n * n / (ph["assembly"] * 1e-3),
@@ -316,6 +330,7 @@ def main():
         "kernels_instrumented_step": {
             k: {"launches": int(v[0]), "ms": v[1], "tflops": (v[2] / (v[1] * 1e-3) / 1e12) if v[1] else 0.0}
             for k, v in kstats_all.items() if v[0] > 0},
+        "adjoint_gradient": adjoint,
         "roofline": roof,
     }
     ctx.close()
