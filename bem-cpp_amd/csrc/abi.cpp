@@ -35,8 +35,6 @@ void kt_resolve(bem_ctx* c) {
     if (k.pending.empty()) return;
     (void)hipStreamSynchronize(c->stream3);
     (void)hipStreamSynchronize(c->stream2);
-    if (c->stream_upd) (void)hipStreamSynchronize(c->stream_upd);
-    if (c->stream_pnl) (void)hipStreamSynchronize(c->stream_pnl);
     (void)hipStreamSynchronize(c->stream);
     for (KernelTimer::Rec& r : k.pending) {
         float ms = 0.0f;
@@ -219,8 +217,6 @@ void bem_destroy(bem_ctx* c) {
     (void)hipStreamSynchronize(c->stream3);
     (void)hipStreamSynchronize(c->stream2);
     (void)hipStreamSynchronize(c->stream);
-    if (c->stream_upd) (void)hipStreamSynchronize(c->stream_upd);
-    if (c->stream_pnl) (void)hipStreamSynchronize(c->stream_pnl);
     free_mesh(c);
     free_matrices(c);
     free_rhs(c);
@@ -230,8 +226,6 @@ void bem_destroy(bem_ctx* c) {
     (void)hipEventDestroy(c->ev[0]);
     (void)hipEventDestroy(c->ev[1]);
     for (hipEvent_t e : c->lu_ev) (void)hipEventDestroy(e);
-    if (c->stream_upd) (void)hipStreamDestroy(c->stream_upd);
-    if (c->stream_pnl) (void)hipStreamDestroy(c->stream_pnl);
     (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream3);
     (void)hipStreamDestroy(c->stream);

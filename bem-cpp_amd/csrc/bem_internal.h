@@ -125,8 +125,6 @@ struct bem_ctx {
     // factorisation stream layout (see factor() in kernels/lu.hip): 0 one
     // stream, 1 panel + update streams on all CUs, 2 CU-partitioned
     int lu_streams = -1;            // -1: not chosen yet
-    int lu_panel_cus = 0;           // CUs reserved for the panel stream (layout 2)
-    hipStream_t stream_upd = nullptr, stream_pnl = nullptr;  // CU-masked streams of layout 2
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t lu_ev[6] = {};  // cross-stream ordering inside the factorisation
     std::string err;

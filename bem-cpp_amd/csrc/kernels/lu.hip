@@ -665,20 +665,13 @@ void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, cons
 // then U12 and the update of those 256 columns; the swaps and the update of
 // all other columns follow on the update stream while panel K+1 is factored.
 // Stream layout (c->lu_streams): 0 = everything on the context stream (no
-// overlap); 1 = panel stream and context stream over all CUs; 2 = the panel
-// stream confined to a reserved set of CUs that the update stream does not
-// use, so the latency-bound panel kernels never share a CU with the trailing
-// GEMM.
+// overlap); 1 = panel stream and context stream over all CUs.
 template <typename T>
 void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWork<T>& w) {
     const int n = c->npad, nb = n / NB;
     const long ld = n;
     hipStream_t us = c->stream, ps = c->stream2, ss = c->stream3;
     if (c->lu_streams == 0) ps = ss = us;
-    if (c->lu_streams == 2) {
-        us = c->stream_upd;
-        ps = c->stream_pnl;
-    }
     hipEvent_t e_panel = c->lu_ev[0], e_next = c->lu_ev[1], e_start = c->lu_ev[2], e_linv = c->lu_ev[3];
     (void)hipEventRecord(e_start, c->stream);  // the matrix and work buffers are ready
     (void)hipStreamWaitEvent(ps, e_start, 0);
@@ -801,13 +794,11 @@ int lu_inverses_launch(bem_ctx* c) {
 
 int lu_setup_streams(bem_ctx* c) {
     if (c->lu_streams >= 0) return 0;
-    // Layout 2 (CU-masked update and panel streams) is switched off: its first
-    // run ended in a host-side crash inside the factorisation whose cause is
-    // not established, so only the one-stream and the overlapping layouts run.
+    // (A layout with the panel stream on CUs reserved by a CU mask ended in a
+    // host-side crash in its first run, cause not established; it was removed.)
     const char* e1 = std::getenv("BEM_LU_STREAMS");
     const int mode = e1 ? std::atoi(e1) : 1;
     c->lu_streams = mode == 0 ? 0 : 1;
-    c->lu_panel_cus = 0;
     return 0;
 }
 
