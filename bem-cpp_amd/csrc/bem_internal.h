@@ -46,6 +46,17 @@ struct RecCoef {
     int deriv;
 };
 
+// Up to three record sets written by one pass over the triangle pairs: the
+// moments (which do not depend on the coefficients, apart from r0 at the
+// R = 0 points of self pairs) are formed once and combined with q[j] into the
+// record array H[j].  All q[j] share `deriv`.
+constexpr int kMaxRecSets = 3;
+struct RecCoefSet {
+    RecCoef q[kMaxRecSets];
+    double* H[kMaxRecSets];
+    int n;
+};
+
 // Per-kernel-class device timing (bem_kernel_stats): HIP events around each
 // launch on the context stream, resolved when the ABI call returns.
 enum KernelClass {
@@ -256,9 +267,14 @@ int rhs_dk_launch(bem_ctx* c, double* dBr, double* dBi);
 int ff_adjoint_launch(bem_ctx* c, const double* dL_dM, double* gr, double* gi, double* explicit_dk);
 // <Z_q, W> = sum_{m,n} Z_q[m][n] W[m][n] over the 2N x 2N matrix whose
 // half-pair records are combined with q (PMCHWT block pattern of Z), W planar
-// FP64 row-major with leading dimension ldw; result (re, im) in out[2].
-int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef& q, const double* Wr, const double* Wi,
-                        long ldw, double* out);
+// FP64 row-major with leading dimension ldw, for the nq coefficient sets q[j]
+// (1 <= nq <= kMaxRecSets, one `deriv`): result j as (re, im) in
+// out[2j], out[2j + 1].  Sets after the first need record arrays of their
+// own: `extra` holds (nq - 1) * asm_record_bytes(c) bytes.
+int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq, double* extra, const double* Wr,
+                        const double* Wi, long ldw, double* out);
+// Bytes of one record array (T (T + 1) / 2 triangle-pair records).
+size_t asm_record_bytes(const bem_ctx* c);
 
 // Small helpers used by several launchers
 int ctx_ensure_rhs(bem_ctx* c, int nrhs);

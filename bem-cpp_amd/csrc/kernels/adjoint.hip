@@ -238,16 +238,30 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad) {
     Cx kappa[3][2] = {{{0, 0}, {ka, 0}}, {{0, 0}, {0, ka}}, {{1, 0}, m}};
     double deta[3][2] = {{0, -m.r / am3}, {0, -m.i / am3}, {0, 0}};
     double dieta[3][2] = {{0, m.r / am}, {0, m.i / am}, {0, 0}};
-    for (int p = 0; p < 3; p++) {
-        Cx contr{0.0, 0.0};
-        for (int pass = 0; pass < 2; pass++) {
-            const RecCoef q = make_coef(pass, k, eta, kappa[p], deta[p], dieta[p]);
-            double out[2];
-            if (int e = asm_contract_launch(c, ap, q, Wr, Wi, npad, out)) return e;
-            contr = cadd(contr, Cx{out[0], out[1]});
-        }
-        grad[p] = -2.0 * contr.r;
+    // One pass per moment kind forms the records of all three parameters
+    // (two extra record arrays); without room for them, one pass per
+    // parameter and kind.
+    DevBuf recs;
+    const size_t rec_bytes = asm_record_bytes(c);
+    const bool merged = hipMalloc(&recs.p, 2 * rec_bytes) == hipSuccess;
+    if (!merged) {
+        recs.p = nullptr;
+        (void)hipGetLastError();
     }
+    Cx contr[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int pass = 0; pass < 2; pass++) {
+        RecCoef q[3];
+        for (int p = 0; p < 3; p++) q[p] = make_coef(pass, k, eta, kappa[p], deta[p], dieta[p]);
+        double out[6];
+        if (merged) {
+            if (int e = asm_contract_launch(c, ap, q, 3, static_cast<double*>(recs.p), Wr, Wi, npad, out)) return e;
+        } else {
+            for (int p = 0; p < 3; p++)
+                if (int e = asm_contract_launch(c, ap, q + p, 1, nullptr, Wr, Wi, npad, out + 2 * p)) return e;
+        }
+        for (int p = 0; p < 3; p++) contr[p] = cadd(contr[p], Cx{out[2 * p], out[2 * p + 1]});
+    }
+    for (int p = 0; p < 3; p++) grad[p] = -2.0 * contr[p].r;
     grad[2] += 2.0 * rhs_term.r + expl_k;
     BEM_HIP(c, hipGetLastError());
     return 0;

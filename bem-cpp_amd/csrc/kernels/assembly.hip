@@ -283,7 +283,7 @@ __device__ __forceinline__ bool touching(const int4& a, const int4& b) {
 template <int NQ, bool LOSSY, int MODE>
 __global__ __launch_bounds__(256) void k_tripair_far(int T, const double4* __restrict__ qp,
                                                      const int4* __restrict__ tv, const double* __restrict__ hc,
-                                                     const double* __restrict__ hp, AsmParams p, RecCoef q,
+                                                     const double* __restrict__ hp, AsmParams p, RecCoefSet q,
                                                      double* __restrict__ H) {
     const int bt = blockIdx.y, bs = blockIdx.x;
     if (bt > bs) return;
@@ -358,10 +358,13 @@ __global__ __launch_bounds__(256) void k_tripair_far(int T, const double4* __res
         mom_fold(mom[0], in0, P);
         mom_fold(mom[1], in1, P);
     }
-    if (MODE == 0)
+    if (MODE == 0) {
         write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
-    else
-        write_record_coef(H, rec_index(t, s, T), mom, q, hc, hp, t, s, nullptr);
+    } else {
+#pragma unroll 1
+        for (int j = 0; j < q.n; j++)
+            write_record_coef(q.H[j], rec_index(t, s, T), mom, q.q[j], hc, hp, t, s, nullptr);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -464,118 +467,131 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
                                                       const double4* __restrict__ qp,
                                                       const double* __restrict__ hc,
                                                       const double* __restrict__ hp,
-                                                      const double4* __restrict__ pv, AsmParams p, RecCoef q,
+                                                      const double4* __restrict__ pv, AsmParams p, RecCoefSet q,
                                                       double* __restrict__ H) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_near) return;
     const int t = pairs[idx].x, s = pairs[idx].y;
     const bool self = t == s;
-    Mom mom[2];
-    mom_zero(mom[0]);
-    mom_zero(mom[1]);
+    // In MODE 2 the R = 0 value of a self pair depends on the coefficient
+    // set, so such a pair forms its moments once per set.
+    const int nmom = (MODE == 2 && self) ? q.n : 1;
 #pragma unroll 1
-    for (int i = 0; i < NQ; i++) {
-        const double4 P = qp[t * NQ + i];
-        Inner in[2];
-        inner_zero(in[0]);
-        inner_zero(in[1]);
+    for (int js = 0; js < nmom; js++) {
+        Mom mom[2];
+        mom_zero(mom[0]);
+        mom_zero(mom[1]);
 #pragma unroll 1
-        for (int j = 0; j < NQ; j++) {
-            const double4 Q = qp[s * NQ + j];
-            const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const double R = sqrt(r2);
+        for (int i = 0; i < NQ; i++) {
+            const double4 P = qp[t * NQ + i];
+            Inner in[2];
+            inner_zero(in[0]);
+            inner_zero(in[1]);
+#pragma unroll 1
+            for (int j = 0; j < NQ; j++) {
+                const double4 Q = qp[s * NQ + j];
+                const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double R = sqrt(r2);
 #pragma unroll
-            for (int kk = 0; kk < 2; kk++) {
-                const double kr = p.kr[kk], ki = p.ki[kk];
-                double Gr, Gi, gr = 0.0, gi = 0.0;
-                if (self && R < 1e-12) {
-                    if (MODE == 2) {
-                        Gr = q.r0[kk][0];
-                        Gi = q.r0[kk][1];
-                    } else {
-                        Gr = 0.0;
-                        Gi = kr * kInv4Pi;
-                    }
-                } else {
-                    double sn, cs;
-                    sincos(kr * R, &sn, &cs);
-                    const double amp = exp(-ki * R) * kInv4Pi / R;
-                    Gr = amp * cs;
-                    Gi = amp * sn;
-                    const bool with_g = !self && R > 1e-6;
-                    if (MODE == 2) {  // dG/dk = iR G (the static 1/(4 pi R) has none), dg/dk = -k G
-                        if (with_g) {
-                            gr = -(kr * Gr - ki * Gi);
-                            gi = -(kr * Gi + ki * Gr);
+                for (int kk = 0; kk < 2; kk++) {
+                    const double kr = p.kr[kk], ki = p.ki[kk];
+                    double Gr, Gi, gr = 0.0, gi = 0.0;
+                    if (self && R < 1e-12) {
+                        if (MODE == 2) {
+                            Gr = q.q[js].r0[kk][0];
+                            Gi = q.q[js].r0[kk][1];
+                        } else {
+                            Gr = 0.0;
+                            Gi = kr * kInv4Pi;
                         }
-                        const double t0 = -R * Gi;
-                        Gi = R * Gr;
-                        Gr = t0;
                     } else {
-                        if (with_g) {
-                            const double ar = (-ki - 1.0 / R) / R, ai = kr / R;
-                            gr = Gr * ar - Gi * ai;
-                            gi = Gr * ai + Gi * ar;
+                        double sn, cs;
+                        sincos(kr * R, &sn, &cs);
+                        const double amp = exp(-ki * R) * kInv4Pi / R;
+                        Gr = amp * cs;
+                        Gi = amp * sn;
+                        const bool with_g = !self && R > 1e-6;
+                        if (MODE == 2) {  // dG/dk = iR G (the static 1/(4 pi R) has none), dg/dk = -k G
+                            if (with_g) {
+                                gr = -(kr * Gr - ki * Gi);
+                                gi = -(kr * Gi + ki * Gr);
+                            }
+                            const double t0 = -R * Gi;
+                            Gi = R * Gr;
+                            Gr = t0;
+                        } else {
+                            if (with_g) {
+                                const double ar = (-ki - 1.0 / R) / R, ai = kr / R;
+                                gr = Gr * ar - Gi * ai;
+                                gi = Gr * ai + Gi * ar;
+                            }
+                            if (self) Gr -= kInv4Pi / R;
                         }
-                        if (self) Gr -= kInv4Pi / R;
                     }
+                    inner_add(in[kk], Q.w, Gr, Gi, gr, gi, Q);
                 }
-                inner_add(in[kk], Q.w, Gr, Gi, gr, gi, Q);
             }
+            mom_fold(mom[0], in[0], P);
+            mom_fold(mom[1], in[1], P);
         }
-        mom_fold(mom[0], in[0], P);
-        mom_fold(mom[1], in[1], P);
-    }
-    if (!self || MODE == 2) {
-        if (MODE == 0)
-            write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
-        else
-            write_record_coef(H, rec_index(t, s, T), mom, q, hc, hp, t, s, nullptr);
-        return;
-    }
-    double SP = 0.0, SPr[3] = {0, 0, 0}, SV[3] = {0, 0, 0}, SrV = 0.0;
-    for (int q = 0; q < NQ; q++) {
-        const double4 x = qp[t * NQ + q];
-        const double4 w = pv[t * NQ + q];  // (P, Vx, Vy, Vz)
-        SP += x.w * w.x;
-        SPr[0] += x.w * w.x * x.x;
-        SPr[1] += x.w * w.x * x.y;
-        SPr[2] += x.w * w.x * x.z;
-        SV[0] += x.w * w.y;
-        SV[1] += x.w * w.z;
-        SV[2] += x.w * w.w;
-        SrV += x.w * (x.x * w.y + x.y * w.z + x.z * w.w);
-    }
-    C2 corr[18];
-    double sv[10];
-    sv[9] = SP;
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) {
-            const double ca = hc[t * 3 + a], cb = hc[t * 3 + b];
-            const double* pa = hp + (t * 3 + a) * 3;
-            const double* pb = hp + (t * 3 + b) * 3;
-            double sym = 0.0, papb = 0.0;
-            for (int e = 0; e < 3; e++) {
-                sym += (pa[e] + pb[e]) * (SPr[e] + SV[e]);
-                papb += pa[e] * pb[e];
+        if (!self || MODE == 2) {
+            if (MODE == 0) {
+                write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+            } else {
+                const int j0 = nmom > 1 ? js : 0, j1 = nmom > 1 ? js + 1 : q.n;
+#pragma unroll 1
+                for (int j = j0; j < j1; j++)
+                    write_record_coef(q.H[j], rec_index(t, s, T), mom, q.q[j], hc, hp, t, s, nullptr);
             }
-            const double vec = SrV - 0.5 * sym + papb * SP;  // real
-            sv[a * 3 + b] = vec;
-            for (int kk = 0; kk < 2; kk++) {
-                const double kr = p.kr[kk], ki = p.ki[kk], k2 = kr * kr + ki * ki;
-                // -(4i/k) SP + ik vec, i/k = (ki + i kr)/|k|^2, ik = -ki + i kr
-                const double re = -4.0 * ki / k2 * SP - ki * vec;
-                const double im = -4.0 * kr / k2 * SP + kr * vec;
-                const double f = ca * cb * kInv4Pi;
-                corr[kk * 9 + a * 3 + b].r = f * re;
-                corr[kk * 9 + a * 3 + b].i = f * im;
-            }
+            continue;
         }
-    if (MODE == 0)
-        write_record(H, rec_index(t, t, T), mom, p, hc, hp, t, t, corr);
-    else
-        write_record_coef(H, rec_index(t, t, T), mom, q, hc, hp, t, t, sv);
+        double SP = 0.0, SPr[3] = {0, 0, 0}, SV[3] = {0, 0, 0}, SrV = 0.0;
+        for (int iq = 0; iq < NQ; iq++) {
+            const double4 x = qp[t * NQ + iq];
+            const double4 w = pv[t * NQ + iq];  // (P, Vx, Vy, Vz)
+            SP += x.w * w.x;
+            SPr[0] += x.w * w.x * x.x;
+            SPr[1] += x.w * w.x * x.y;
+            SPr[2] += x.w * w.x * x.z;
+            SV[0] += x.w * w.y;
+            SV[1] += x.w * w.z;
+            SV[2] += x.w * w.w;
+            SrV += x.w * (x.x * w.y + x.y * w.z + x.z * w.w);
+        }
+        C2 corr[18];
+        double sv[10];
+        sv[9] = SP;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                const double ca = hc[t * 3 + a], cb = hc[t * 3 + b];
+                const double* pa = hp + (t * 3 + a) * 3;
+                const double* pb = hp + (t * 3 + b) * 3;
+                double sym = 0.0, papb = 0.0;
+                for (int e = 0; e < 3; e++) {
+                    sym += (pa[e] + pb[e]) * (SPr[e] + SV[e]);
+                    papb += pa[e] * pb[e];
+                }
+                const double vec = SrV - 0.5 * sym + papb * SP;  // real
+                sv[a * 3 + b] = vec;
+                for (int kk = 0; kk < 2; kk++) {
+                    const double kr = p.kr[kk], ki = p.ki[kk], k2 = kr * kr + ki * ki;
+                    // -(4i/k) SP + ik vec, i/k = (ki + i kr)/|k|^2, ik = -ki + i kr
+                    const double re = -4.0 * ki / k2 * SP - ki * vec;
+                    const double im = -4.0 * kr / k2 * SP + kr * vec;
+                    const double f = ca * cb * kInv4Pi;
+                    corr[kk * 9 + a * 3 + b].r = f * re;
+                    corr[kk * 9 + a * 3 + b].i = f * im;
+                }
+            }
+        if (MODE == 0) {
+            write_record(H, rec_index(t, t, T), mom, p, hc, hp, t, t, corr);
+        } else {
+#pragma unroll 1
+            for (int j = 0; j < q.n; j++)
+                write_record_coef(q.H[j], rec_index(t, t, T), mom, q.q[j], hc, hp, t, t, sv);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -641,20 +657,30 @@ __global__ void k_pad(int n, int ld, F* __restrict__ Ar, F* __restrict__ Ai) {
 }
 
 // <Z_H, W> over the 2N x 2N matrix whose entries k_gather would form from the
-// records: thread (m, n) sums its four PMCHWT entries times W, the block sums
-// in a fixed tree order into part[(m * gridDim.x + blockIdx.x) * 2 + {0, 1}].
-__global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __restrict__ rwg_half,
-                                                  const double* __restrict__ H, const double* __restrict__ Wr,
-                                                  const double* __restrict__ Wi, long ldw, double* __restrict__ part) {
-    __shared__ double red[2][256];
+// records, for the NS record arrays H.p[j] at once (W is read once): thread
+// (m, n) sums its four PMCHWT entries times W, the block sums in a fixed tree
+// order into part[((m * gridDim.x + blockIdx.x) * NS + j) * 2 + {0, 1}].
+struct RecArrays {
+    const double* p[kMaxRecSets];
+};
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __restrict__ rwg_half, RecArrays H,
+                                                  const double* __restrict__ Wr, const double* __restrict__ Wi,
+                                                  long ldw, double* __restrict__ part) {
+    __shared__ double red[2 * NS][256];
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = blockIdx.y;
-    double acc[2] = {0.0, 0.0};
+    double acc[NS][2];
+#pragma unroll
+    for (int j = 0; j < NS; j++) acc[j][0] = acc[j][1] = 0.0;
     if (n < N) {
         const int4 hm = rwg_half[m], hn = rwg_half[n];
         const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
         const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
-        double P[2] = {0, 0}, Q[2] = {0, 0}, K[2] = {0, 0};
+        double P[NS][2], Q[NS][2], K[NS][2];
+#pragma unroll
+        for (int j = 0; j < NS; j++) P[j][0] = P[j][1] = Q[j][0] = Q[j][1] = K[j][0] = K[j][1] = 0.0;
 #pragma unroll
         for (int sa = 0; sa < 2; sa++)
 #pragma unroll
@@ -669,63 +695,62 @@ __global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __re
                     rec = rec_index(tb, ta, T);
                     slot = ln[sb] * 3 + lm[sa];
                 }
-                const double2* h = reinterpret_cast<const double2*>(H + rec * kRec + slot * 6);
-                const double2 a = h[0], b = h[1], c = h[2];
-                P[0] += a.x; P[1] += a.y;
-                Q[0] += b.x; Q[1] += b.y;
-                K[0] += c.x; K[1] += c.y;
+#pragma unroll
+                for (int j = 0; j < NS; j++) {
+                    const double2* h = reinterpret_cast<const double2*>(H.p[j] + rec * kRec + slot * 6);
+                    const double2 a = h[0], b = h[1], c = h[2];
+                    P[j][0] += a.x; P[j][1] += a.y;
+                    Q[j][0] += b.x; Q[j][1] += b.y;
+                    K[j][0] += c.x; K[j][1] += c.y;
+                }
             }
         const size_t r0 = (size_t)m * ldw, r1 = (size_t)(N + m) * ldw;
-        const double mK[2] = {-K[0], -K[1]};
-        cmac(acc, P, Wr[r0 + n], Wi[r0 + n]);
-        cmac(acc, mK, Wr[r0 + N + n], Wi[r0 + N + n]);
-        cmac(acc, K, Wr[r1 + n], Wi[r1 + n]);
-        cmac(acc, Q, Wr[r1 + N + n], Wi[r1 + N + n]);
+        const double w00r = Wr[r0 + n], w00i = Wi[r0 + n], w01r = Wr[r0 + N + n], w01i = Wi[r0 + N + n];
+        const double w10r = Wr[r1 + n], w10i = Wi[r1 + n], w11r = Wr[r1 + N + n], w11i = Wi[r1 + N + n];
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            const double mK[2] = {-K[j][0], -K[j][1]};
+            cmac(acc[j], P[j], w00r, w00i);
+            cmac(acc[j], mK, w01r, w01i);
+            cmac(acc[j], K[j], w10r, w10i);
+            cmac(acc[j], Q[j], w11r, w11i);
+        }
     }
-    red[0][threadIdx.x] = acc[0];
-    red[1][threadIdx.x] = acc[1];
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        red[2 * j][threadIdx.x] = acc[j][0];
+        red[2 * j + 1][threadIdx.x] = acc[j][1];
+    }
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + w];
-            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+#pragma unroll
+            for (int c = 0; c < 2 * NS; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const long o = ((long)m * gridDim.x + blockIdx.x) * 2;
-        part[o] = red[0][0];
-        part[o + 1] = red[1][0];
-    }
+    if (threadIdx.x < 2 * NS) part[((long)m * gridDim.x + blockIdx.x) * 2 * NS + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out[0..1] = sum of the np complex partials, one block, fixed order.
-__global__ __launch_bounds__(256) void k_sum_parts(long np, const double* __restrict__ part, double* __restrict__ out) {
-    __shared__ double red[2][256];
-    double a = 0.0, b = 0.0;
-    for (long i = threadIdx.x; i < np; i += 256) {
-        a += part[2 * i];
-        b += part[2 * i + 1];
-    }
-    red[0][threadIdx.x] = a;
-    red[1][threadIdx.x] = b;
+// out[c] = sum over i < np of part[i * ncomp + c], block c, fixed order.
+__global__ __launch_bounds__(256) void k_sum_parts(long np, int ncomp, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    double a = 0.0;
+    for (long i = threadIdx.x; i < np; i += 256) a += part[i * ncomp + c];
+    red[threadIdx.x] = a;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + w];
-            red[1][threadIdx.x] += red[1][threadIdx.x + w];
-        }
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out[0] = red[0][0];
-        out[1] = red[1][0];
-    }
+    if (threadIdx.x == 0) out[c] = red[0];
 }
 
 // Graglia data (modes 0 and 1) and the records of all triangle pairs into c->H.
 template <int NQ, int MODE>
-void launch_records(bem_ctx* c, const AsmParams& p, const RecCoef& q, bool lossy, const double* dverts,
+void launch_records(bem_ctx* c, const AsmParams& p, const RecCoefSet& q, bool lossy, const double* dverts,
                     const double* dqw) {
     const DevMesh& d = c->dm;
     const int T = d.T;
@@ -754,7 +779,7 @@ int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi
               const double* dverts, const double* dqw) {
     const DevMesh& d = c->dm;
     const int T = d.T;
-    launch_records<NQ, 0>(c, p, RecCoef{}, lossy, dverts, dqw);
+    launch_records<NQ, 0>(c, p, RecCoefSet{}, lossy, dverts, dqw);
     const int N = d.N;
     const int kt = kt_mark(c, KS_GATHER);
     hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256, N), dim3(256), 0, c->stream, N, T, c->npad, d.rwg_half, c->H,
@@ -827,31 +852,57 @@ int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
     return 0;
 }
 
-int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef& q, const double* Wr, const double* Wi,
-                        long ldw, double* out) {
+size_t asm_record_bytes(const bem_ctx* c) {
+    const size_t T = (size_t)c->dm.T;
+    return T * (T + 1) / 2 * kRec * sizeof(double);
+}
+
+int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq, double* extra, const double* Wr,
+                        const double* Wi, long ldw, double* out) {
+    if (nq < 1 || nq > kMaxRecSets) return bem_fail(c, BEM_ERR_ARG, "contraction: 1..3 coefficient sets");
+    for (int j = 1; j < nq; j++)
+        if (q[j].deriv != q[0].deriv) return bem_fail(c, BEM_ERR_ARG, "contraction: sets of one moment kind");
     const double *dverts, *dqw;
     if (int rc = asm_prepare(c, &dverts, &dqw)) return rc;
+    RecCoefSet qs{};
+    qs.n = nq;
+    const size_t rec_doubles = asm_record_bytes(c) / sizeof(double);
+    RecArrays ra{};
+    for (int j = 0; j < nq; j++) {
+        qs.q[j] = q[j];
+        qs.H[j] = j == 0 ? c->H : extra + (size_t)(j - 1) * rec_doubles;
+        ra.p[j] = qs.H[j];
+    }
     const bool lossy = p.ki[1] != 0.0;
     const int Nq = c->dm.Nq;
-    if (q.deriv) {
-        if (Nq == 4) launch_records<4, 2>(c, p, q, lossy, dverts, dqw);
-        else if (Nq == 13) launch_records<13, 2>(c, p, q, lossy, dverts, dqw);
-        else launch_records<7, 2>(c, p, q, lossy, dverts, dqw);
+    if (q[0].deriv) {
+        if (Nq == 4) launch_records<4, 2>(c, p, qs, lossy, dverts, dqw);
+        else if (Nq == 13) launch_records<13, 2>(c, p, qs, lossy, dverts, dqw);
+        else launch_records<7, 2>(c, p, qs, lossy, dverts, dqw);
     } else {
-        if (Nq == 4) launch_records<4, 1>(c, p, q, lossy, dverts, dqw);
-        else if (Nq == 13) launch_records<13, 1>(c, p, q, lossy, dverts, dqw);
-        else launch_records<7, 1>(c, p, q, lossy, dverts, dqw);
+        if (Nq == 4) launch_records<4, 1>(c, p, qs, lossy, dverts, dqw);
+        else if (Nq == 13) launch_records<13, 1>(c, p, qs, lossy, dverts, dqw);
+        else launch_records<7, 1>(c, p, qs, lossy, dverts, dqw);
     }
     const int N = c->dm.N;
     const int gx = (N + 255) / 256;
     const long np = (long)gx * N;
+    const int ncomp = 2 * nq;
     double* part = nullptr;
-    if (hipMalloc(&part, (np * 2 + 2) * sizeof(double)) != hipSuccess)
+    if (hipMalloc(&part, (np + 1) * ncomp * sizeof(double)) != hipSuccess)
         return bem_fail(c, BEM_ERR_NOMEM, "contraction partial sums");
-    hipLaunchKernelGGL(k_contract, dim3(gx, N), dim3(256), 0, c->stream, N, c->dm.T, c->dm.rwg_half, c->H, Wr, Wi,
-                       ldw, part);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, np, part, part + np * 2);
-    const hipError_t e1 = hipMemcpyAsync(out, part + np * 2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    const dim3 grid(gx, N);
+    const int4* rh = c->dm.rwg_half;
+    const int T = c->dm.T;
+    if (nq == 1)
+        hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
+    else if (nq == 2)
+        hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
+    else
+        hipLaunchKernelGGL(k_contract<3>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(ncomp), dim3(256), 0, c->stream, np, ncomp, part, part + np * ncomp);
+    const hipError_t e1 =
+        hipMemcpyAsync(out, part + np * ncomp, ncomp * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     const hipError_t e2 = hipStreamSynchronize(c->stream);
     (void)hipFree(part);
     if (e1 != hipSuccess || e2 != hipSuccess)
