@@ -657,30 +657,61 @@ __global__ void k_pad(int n, int ld, F* __restrict__ Ar, F* __restrict__ Ai) {
 }
 
 // <Z_H, W> over the 2N x 2N matrix whose entries k_gather would form from the
-// records, for the NS record arrays H.p[j] at once (W is read once): thread
-// (m, n) sums its four PMCHWT entries times W, the block sums in a fixed tree
-// order into part[((m * gridDim.x + blockIdx.x) * NS + j) * 2 + {0, 1}].
+// records, for the NS record arrays H.p[j] at once.  The records give Z the
+// symmetry Z[n][m] = Z[m][n] within the P and Q blocks and Ks[n][m] =
+// Ks[m][n], so one read of the four half-pair records of (m, n), m < n,
+// serves both (m, n) and (n, m):
+//   <Z, W> = sum_{m <= n} P Ws00 - Ks Ws01 + Ks Ws10 + Q Ws11,
+//   Ws_b = W_b[m][n] + W_b[n][m] (m < n), W_b[m][m] (m = n),
+// with W_b the four N x N blocks of W.  Block = 16 x 16 tile of (m, n) with
+// the m tile at or before the n tile; W_b[n][m] comes through LDS so both W
+// reads are row segments.  Each block sums in a fixed tree order into
+// part[(tile * NS + j) * 2 + {0, 1}], tile = blockIdx.y * gridDim.x +
+// blockIdx.x (tiles below the diagonal write zeros).
 struct RecArrays {
     const double* p[kMaxRecSets];
 };
+
+constexpr int kCt = 16;  // contraction tile edge (RWGs)
 
 template <int NS>
 __global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __restrict__ rwg_half, RecArrays H,
                                                   const double* __restrict__ Wr, const double* __restrict__ Wi,
                                                   long ldw, double* __restrict__ part) {
+    __shared__ double wt[8][kCt][kCt + 1];  // W_b[n][m], b = 0..3, (re, im)
     __shared__ double red[2 * NS][256];
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int m = blockIdx.y;
+    const int tm0 = blockIdx.y * kCt, tn0 = blockIdx.x * kCt;
+    const long tile = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    if (blockIdx.y > blockIdx.x) {
+        if (threadIdx.x < 2 * NS) part[tile * 2 * NS + threadIdx.x] = 0.0;
+        return;
+    }
+    const int i = threadIdx.x / kCt, j = threadIdx.x % kCt;
+    {  // W_b[n][m] for n = tn0 + i, m = tm0 + j, row segments
+        const int n = tn0 + i, m = tm0 + j;
+        const bool ok = n < N && m < N;
+        const size_t rn = (size_t)n * ldw, rN = (size_t)(N + n) * ldw;
+        wt[0][i][j] = ok ? Wr[rn + m] : 0.0;
+        wt[1][i][j] = ok ? Wi[rn + m] : 0.0;
+        wt[2][i][j] = ok ? Wr[rn + N + m] : 0.0;
+        wt[3][i][j] = ok ? Wi[rn + N + m] : 0.0;
+        wt[4][i][j] = ok ? Wr[rN + m] : 0.0;
+        wt[5][i][j] = ok ? Wi[rN + m] : 0.0;
+        wt[6][i][j] = ok ? Wr[rN + N + m] : 0.0;
+        wt[7][i][j] = ok ? Wi[rN + N + m] : 0.0;
+    }
+    __syncthreads();
+    const int m = tm0 + i, n = tn0 + j;
     double acc[NS][2];
 #pragma unroll
-    for (int j = 0; j < NS; j++) acc[j][0] = acc[j][1] = 0.0;
-    if (n < N) {
+    for (int q = 0; q < NS; q++) acc[q][0] = acc[q][1] = 0.0;
+    if (m < N && n < N && m <= n) {
         const int4 hm = rwg_half[m], hn = rwg_half[n];
         const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
         const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
         double P[NS][2], Q[NS][2], K[NS][2];
 #pragma unroll
-        for (int j = 0; j < NS; j++) P[j][0] = P[j][1] = Q[j][0] = Q[j][1] = K[j][0] = K[j][1] = 0.0;
+        for (int q = 0; q < NS; q++) P[q][0] = P[q][1] = Q[q][0] = Q[q][1] = K[q][0] = K[q][1] = 0.0;
 #pragma unroll
         for (int sa = 0; sa < 2; sa++)
 #pragma unroll
@@ -696,30 +727,33 @@ __global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __re
                     slot = ln[sb] * 3 + lm[sa];
                 }
 #pragma unroll
-                for (int j = 0; j < NS; j++) {
-                    const double2* h = reinterpret_cast<const double2*>(H.p[j] + rec * kRec + slot * 6);
+                for (int q = 0; q < NS; q++) {
+                    const double2* h = reinterpret_cast<const double2*>(H.p[q] + rec * kRec + slot * 6);
                     const double2 a = h[0], b = h[1], c = h[2];
-                    P[j][0] += a.x; P[j][1] += a.y;
-                    Q[j][0] += b.x; Q[j][1] += b.y;
-                    K[j][0] += c.x; K[j][1] += c.y;
+                    P[q][0] += a.x; P[q][1] += a.y;
+                    Q[q][0] += b.x; Q[q][1] += b.y;
+                    K[q][0] += c.x; K[q][1] += c.y;
                 }
             }
         const size_t r0 = (size_t)m * ldw, r1 = (size_t)(N + m) * ldw;
-        const double w00r = Wr[r0 + n], w00i = Wi[r0 + n], w01r = Wr[r0 + N + n], w01i = Wi[r0 + N + n];
-        const double w10r = Wr[r1 + n], w10i = Wi[r1 + n], w11r = Wr[r1 + N + n], w11i = Wi[r1 + N + n];
+        const double f = m < n ? 1.0 : 0.0;  // the diagonal counts once
+        const double w00r = Wr[r0 + n] + f * wt[0][j][i], w00i = Wi[r0 + n] + f * wt[1][j][i];
+        const double w01r = Wr[r0 + N + n] + f * wt[2][j][i], w01i = Wi[r0 + N + n] + f * wt[3][j][i];
+        const double w10r = Wr[r1 + n] + f * wt[4][j][i], w10i = Wi[r1 + n] + f * wt[5][j][i];
+        const double w11r = Wr[r1 + N + n] + f * wt[6][j][i], w11i = Wi[r1 + N + n] + f * wt[7][j][i];
 #pragma unroll
-        for (int j = 0; j < NS; j++) {
-            const double mK[2] = {-K[j][0], -K[j][1]};
-            cmac(acc[j], P[j], w00r, w00i);
-            cmac(acc[j], mK, w01r, w01i);
-            cmac(acc[j], K[j], w10r, w10i);
-            cmac(acc[j], Q[j], w11r, w11i);
+        for (int q = 0; q < NS; q++) {
+            const double mK[2] = {-K[q][0], -K[q][1]};
+            cmac(acc[q], P[q], w00r, w00i);
+            cmac(acc[q], mK, w01r, w01i);
+            cmac(acc[q], K[q], w10r, w10i);
+            cmac(acc[q], Q[q], w11r, w11i);
         }
     }
 #pragma unroll
-    for (int j = 0; j < NS; j++) {
-        red[2 * j][threadIdx.x] = acc[j][0];
-        red[2 * j + 1][threadIdx.x] = acc[j][1];
+    for (int q = 0; q < NS; q++) {
+        red[2 * q][threadIdx.x] = acc[q][0];
+        red[2 * q + 1][threadIdx.x] = acc[q][1];
     }
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -729,19 +763,19 @@ __global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __re
         }
         __syncthreads();
     }
-    if (threadIdx.x < 2 * NS) part[((long)m * gridDim.x + blockIdx.x) * 2 * NS + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 2 * NS) part[tile * 2 * NS + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // out[c] = sum over i < np of part[i * ncomp + c], block c, fixed order.
-__global__ __launch_bounds__(256) void k_sum_parts(long np, int ncomp, const double* __restrict__ part,
-                                                   double* __restrict__ out) {
-    __shared__ double red[256];
+__global__ __launch_bounds__(1024) void k_sum_parts(long np, int ncomp, const double* __restrict__ part,
+                                                    double* __restrict__ out) {
+    __shared__ double red[1024];
     const int c = blockIdx.x;
     double a = 0.0;
-    for (long i = threadIdx.x; i < np; i += 256) a += part[i * ncomp + c];
+    for (long i = threadIdx.x; i < np; i += 1024) a += part[i * ncomp + c];
     red[threadIdx.x] = a;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 512; w > 0; w >>= 1) {
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -885,13 +919,13 @@ int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq
         else launch_records<7, 1>(c, p, qs, lossy, dverts, dqw);
     }
     const int N = c->dm.N;
-    const int gx = (N + 255) / 256;
-    const long np = (long)gx * N;
+    const int gx = (N + kCt - 1) / kCt;
+    const long np = (long)gx * gx;
     const int ncomp = 2 * nq;
     double* part = nullptr;
     if (hipMalloc(&part, (np + 1) * ncomp * sizeof(double)) != hipSuccess)
         return bem_fail(c, BEM_ERR_NOMEM, "contraction partial sums");
-    const dim3 grid(gx, N);
+    const dim3 grid(gx, gx);
     const int4* rh = c->dm.rwg_half;
     const int T = c->dm.T;
     if (nq == 1)
@@ -900,7 +934,7 @@ int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq
         hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
     else
         hipLaunchKernelGGL(k_contract<3>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
-    hipLaunchKernelGGL(k_sum_parts, dim3(ncomp), dim3(256), 0, c->stream, np, ncomp, part, part + np * ncomp);
+    hipLaunchKernelGGL(k_sum_parts, dim3(ncomp), dim3(1024), 0, c->stream, np, ncomp, part, part + np * ncomp);
     const hipError_t e1 =
         hipMemcpyAsync(out, part + np * ncomp, ncomp * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     const hipError_t e2 = hipStreamSynchronize(c->stream);
