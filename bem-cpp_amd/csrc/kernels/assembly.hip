@@ -32,8 +32,8 @@
 //   k_graglia      P(r_q), V(r_q) at every triangle's own points      (FP64)
 //   k_tripair_far  non-touching triangle pairs -> H records   (FP32 trig, FP64 sums)
 //   k_tripair_near touching pairs incl. self    -> H records   (FP64)
-//   k_gather       Z[m][n] = sum of the 4 half-pair records of (m, n) in a
-//                  fixed order; PMCHWT blocks formed in place, FP64 and FP32
+//   k_gather       Z[m][n] = Z[n][m] = sum of the 4 half-pair records of (m, n)
+//                  in a fixed order; PMCHWT blocks formed in place, FP64 and FP32
 //   k_pad          identity on the padding rows/columns
 // H record of pair (t <= s): 9 half pairs (a on t, b on s) x {P, Q, Ks} complex,
 // P = eta_e L_e + eta_i L_i, Q = L_e/eta_e + L_i/eta_i, Ks = K_e + K_i.
@@ -597,51 +597,85 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
 // ---------------------------------------------------------------------------
 // Z[m][n] from the four half-pair records of RWGs m and n, PMCHWT blocks:
 // Z[m][n] = P, Z[m][N+n] = -Ks, Z[N+m][n] = Ks, Z[N+m][N+n] = Q.
-// One thread per (m, n); n fastest for coalesced row writes.
+// The records are shared by (m, n) and (n, m) (P, Q and Ks are symmetric in
+// the RWG pair), so one read of them gives both: block = 16 x 16 tile of
+// (m, n) with the m tile at or before the n tile, thread (i, j) sums the pair
+// (tm0 + i, tn0 + j) when m <= n and writes its row segment; the (n, m)
+// entries go out as row segments of the mirrored tile through LDS.
 // ---------------------------------------------------------------------------
+constexpr int kGt = 16;  // gather tile edge (RWGs)
+
 __global__ __launch_bounds__(256) void k_gather(int N, int T, int ld, const int4* __restrict__ rwg_half,
                                                 const double* __restrict__ H, double* __restrict__ Zr,
                                                 double* __restrict__ Zi, float* __restrict__ Wr,
                                                 float* __restrict__ Wi) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int m = blockIdx.y;
-    if (n >= N) return;
-    const int4 hm = rwg_half[m], hn = rwg_half[n];
-    const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
-    const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
+    if (blockIdx.y > blockIdx.x) return;
+    __shared__ double sv[6][kGt][kGt + 1];  // (P, Q, Ks) x (re, im) of pair (i, j)
+    const int tm0 = blockIdx.y * kGt, tn0 = blockIdx.x * kGt;
+    const int i = threadIdx.x / kGt, j = threadIdx.x % kGt;
+    const int m = tm0 + i, n = tn0 + j;
+    const bool own = m < N && n < N && m <= n;
     double P0 = 0, P1 = 0, Q0 = 0, Q1 = 0, K0 = 0, K1 = 0;
+    if (own) {
+        const int4 hm = rwg_half[m], hn = rwg_half[n];
+        const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
+        const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
 #pragma unroll
-    for (int sa = 0; sa < 2; sa++)
+        for (int sa = 0; sa < 2; sa++)
 #pragma unroll
-        for (int sb = 0; sb < 2; sb++) {
-            const int ta = tm[sa], tb = tn[sb];
-            long long rec;
-            int slot;
-            if (ta <= tb) {
-                rec = rec_index(ta, tb, T);
-                slot = lm[sa] * 3 + ln[sb];
-            } else {
-                rec = rec_index(tb, ta, T);
-                slot = ln[sb] * 3 + lm[sa];
+            for (int sb = 0; sb < 2; sb++) {
+                const int ta = tm[sa], tb = tn[sb];
+                long long rec;
+                int slot;
+                if (ta <= tb) {
+                    rec = rec_index(ta, tb, T);
+                    slot = lm[sa] * 3 + ln[sb];
+                } else {
+                    rec = rec_index(tb, ta, T);
+                    slot = ln[sb] * 3 + lm[sa];
+                }
+                const double2* h = reinterpret_cast<const double2*>(H + rec * kRec + slot * 6);
+                const double2 a = h[0], b = h[1], c = h[2];
+                P0 += a.x; P1 += a.y;
+                Q0 += b.x; Q1 += b.y;
+                K0 += c.x; K1 += c.y;
             }
-            const double2* h = reinterpret_cast<const double2*>(H + rec * kRec + slot * 6);
-            const double2 a = h[0], b = h[1], c = h[2];
-            P0 += a.x; P1 += a.y;
-            Q0 += b.x; Q1 += b.y;
-            K0 += c.x; K1 += c.y;
+        const size_t r0 = (size_t)m * ld, r1 = (size_t)(N + m) * ld;
+        if (Zr) {
+            Zr[r0 + n] = P0;      Zi[r0 + n] = P1;
+            Zr[r0 + N + n] = -K0; Zi[r0 + N + n] = -K1;
+            Zr[r1 + n] = K0;      Zi[r1 + n] = K1;
+            Zr[r1 + N + n] = Q0;  Zi[r1 + N + n] = Q1;
         }
-    const size_t r0 = (size_t)m * ld, r1 = (size_t)(N + m) * ld;
-    if (Zr) {
-        Zr[r0 + n] = P0;      Zi[r0 + n] = P1;
-        Zr[r0 + N + n] = -K0; Zi[r0 + N + n] = -K1;
-        Zr[r1 + n] = K0;      Zi[r1 + n] = K1;
-        Zr[r1 + N + n] = Q0;  Zi[r1 + N + n] = Q1;
+        if (Wr) {
+            Wr[r0 + n] = (float)P0;      Wi[r0 + n] = (float)P1;
+            Wr[r0 + N + n] = (float)-K0; Wi[r0 + N + n] = (float)-K1;
+            Wr[r1 + n] = (float)K0;      Wi[r1 + n] = (float)K1;
+            Wr[r1 + N + n] = (float)Q0;  Wi[r1 + N + n] = (float)Q1;
+        }
     }
-    if (Wr) {
-        Wr[r0 + n] = (float)P0;      Wi[r0 + n] = (float)P1;
-        Wr[r0 + N + n] = (float)-K0; Wi[r0 + N + n] = (float)-K1;
-        Wr[r1 + n] = (float)K0;      Wi[r1 + n] = (float)K1;
-        Wr[r1 + N + n] = (float)Q0;  Wi[r1 + N + n] = (float)Q1;
+    sv[0][i][j] = P0; sv[1][i][j] = P1;
+    sv[2][i][j] = Q0; sv[3][i][j] = Q1;
+    sv[4][i][j] = K0; sv[5][i][j] = K1;
+    __syncthreads();
+    // mirrored entry: row n = tn0 + i, column m = tm0 + j, from pair (j, i)
+    const int mm = tm0 + j, nn = tn0 + i;
+    if (mm < N && nn < N && mm < nn) {
+        const double p0 = sv[0][j][i], p1 = sv[1][j][i], q0 = sv[2][j][i], q1 = sv[3][j][i];
+        const double k0 = sv[4][j][i], k1 = sv[5][j][i];
+        const size_t r0 = (size_t)nn * ld, r1 = (size_t)(N + nn) * ld;
+        if (Zr) {
+            Zr[r0 + mm] = p0;      Zi[r0 + mm] = p1;
+            Zr[r0 + N + mm] = -k0; Zi[r0 + N + mm] = -k1;
+            Zr[r1 + mm] = k0;      Zi[r1 + mm] = k1;
+            Zr[r1 + N + mm] = q0;  Zi[r1 + N + mm] = q1;
+        }
+        if (Wr) {
+            Wr[r0 + mm] = (float)p0;      Wi[r0 + mm] = (float)p1;
+            Wr[r0 + N + mm] = (float)-k0; Wi[r0 + N + mm] = (float)-k1;
+            Wr[r1 + mm] = (float)k0;      Wi[r1 + mm] = (float)k1;
+            Wr[r1 + N + mm] = (float)q0;  Wi[r1 + N + mm] = (float)q1;
+        }
     }
 }
 
@@ -816,9 +850,11 @@ int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi
     launch_records<NQ, 0>(c, p, RecCoefSet{}, lossy, dverts, dqw);
     const int N = d.N;
     const int kt = kt_mark(c, KS_GATHER);
-    hipLaunchKernelGGL(k_gather, dim3((N + 255) / 256, N), dim3(256), 0, c->stream, N, T, c->npad, d.rwg_half, c->H,
+    const int gt = (N + kGt - 1) / kGt;
+    hipLaunchKernelGGL(k_gather, dim3(gt, gt), dim3(256), 0, c->stream, N, T, c->npad, d.rwg_half, c->H,
                        Zr, Zi, Wr, Wi);
-    kt_close(c, kt, 0.0, (double)N * N * (4.0 * 48.0 + (Zr ? 64.0 : 0.0) + (Wr ? 32.0 : 0.0)));
+    // records of each unordered RWG pair read once, every Z entry written once
+    kt_close(c, kt, 0.0, (double)N * N * (2.0 * 48.0 + (Zr ? 64.0 : 0.0) + (Wr ? 32.0 : 0.0)));
     const int n = 2 * N;
     if (c->npad > n) {
         if (Zr)
