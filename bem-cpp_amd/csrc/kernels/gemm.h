@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
+#include <type_traits>
 
 constexpr int GEMM_MODE_URGENT = 4;
 
@@ -574,4 +576,142 @@ inline int cgemm_splitk_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const 
     splitk_launch<T>(c, cfg, S, M, N, K, Ar, Ai, lda, Br, Bi, ldb, P, Cr, Ci, ldc, mode, role);
     kt_close(c, k, fl, b);
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Short-K products, K = 4 * kShortKW: C (M x N) op= A B with one 16 x 16
+// output tile per block and the block's four waves each taking a quarter of
+// K.  The triangular solves' diagonal-block products (256 x ncol x 256) and
+// their last updates have too few output tiles and too short a K for the
+// LDS-staged kernel to hide its load latency; here every wave issues all its
+// loads at once, straight into MFMA operand registers.  Lane (r, g) =
+// (lane % 16, lane / 16) holds A[m0 + r][k + 4g + i] (one 16-byte row chunk)
+// and B[k + 4g + i][n0 + r] for MFMA i = 0..3 of each 16-wide K step: A and
+// B see the same order of k.  The four wave tiles are summed through LDS in
+// wave order.  Blocks b with one value of b % 8 (one XCD) take a contiguous
+// run of the row-major tile order, so the column tiles of a row share that
+// XCD's L2 copy of A's rows.  FP32: 3M form; FP64: 4M form (as k_cgemm).
+// ---------------------------------------------------------------------------
+constexpr int kShortKW = 64;  // K per wave
+
+template <typename T>
+__device__ __forceinline__ void load_row4(const T* __restrict__ p, T o[4]);
+template <>
+__device__ __forceinline__ void load_row4<float>(const float* __restrict__ p, float o[4]) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+}
+template <>
+__device__ __forceinline__ void load_row4<double>(const double* __restrict__ p, double o[4]) {
+    const double2 v0 = *reinterpret_cast<const double2*>(p), v1 = *reinterpret_cast<const double2*>(p + 2);
+    o[0] = v0.x; o[1] = v0.y; o[2] = v1.x; o[3] = v1.y;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_cgemm_short(int M, int N, const T* __restrict__ Ar, const T* __restrict__ Ai,
+                                                     long lda, const T* __restrict__ Br, const T* __restrict__ Bi,
+                                                     long ldb, T* __restrict__ Cr, T* __restrict__ Ci, long ldc,
+                                                     int mode) {
+    constexpr bool M3 = sizeof(T) == 4;
+    constexpr int NACC = M3 ? 3 : 2;
+    constexpr int NS = kShortKW / 16;  // K steps per wave
+    typedef typename std::conditional<M3, v4f, v4d>::type acc_t;
+    __shared__ T red[4][2][16][17];
+    const int nct = N / 16, G = (M / 16) * nct;
+    const int per = (G + 7) / 8;
+    const int f = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (f >= G) return;
+    const int m0 = (f / nct) * 16, n0 = (f % nct) * 16;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    T ar[NS][4], ai[NS][4], br[NS][4], bi[NS][4];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int k = w * kShortKW + 16 * s + 4 * g;
+        const long oa = (long)(m0 + r) * lda + k;
+        load_row4<T>(Ar + oa, ar[s]);
+        load_row4<T>(Ai + oa, ai[s]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const long ob = (long)(k + i) * ldb + n0 + r;
+            br[s][i] = Br[ob];
+            bi[s][i] = Bi[ob];
+        }
+    }
+    // every load in flight before the first MFMA waits on its operands
+    __builtin_amdgcn_sched_barrier(0);
+    acc_t acc[NACC];
+#pragma unroll
+    for (int a = 0; a < NACC; a++) acc[a] = acc_t{};
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if constexpr (M3) {
+                acc[0] = mfma16x16x4(ar[s][i], br[s][i], acc[0]);
+                acc[1] = mfma16x16x4(ai[s][i], bi[s][i], acc[1]);
+                acc[2] = mfma16x16x4(ar[s][i] + ai[s][i], br[s][i] + bi[s][i], acc[2]);
+            } else {
+                acc[0] = mfma16x16x4(ar[s][i], br[s][i], acc[0]);
+                acc[0] = mfma16x16x4(ai[s][i], -bi[s][i], acc[0]);
+                acc[1] = mfma16x16x4(ar[s][i], bi[s][i], acc[1]);
+                acc[1] = mfma16x16x4(ai[s][i], br[s][i], acc[1]);
+            }
+        }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int row = M3 ? 4 * g + q : g + 4 * q;
+        if constexpr (M3) {
+            red[w][0][row][r] = acc[0][q] - acc[1][q];
+            red[w][1][row][r] = acc[2][q] - acc[0][q] - acc[1][q];
+        } else {
+            red[w][0][row][r] = acc[0][q];
+            red[w][1][row][r] = acc[1][q];
+        }
+    }
+    __syncthreads();
+    const int row = threadIdx.x / 16, col = threadIdx.x % 16;
+    const T vr = ((red[0][0][row][col] + red[1][0][row][col]) + red[2][0][row][col]) + red[3][0][row][col];
+    const T vi = ((red[0][1][row][col] + red[1][1][row][col]) + red[2][1][row][col]) + red[3][1][row][col];
+    const long o = (long)(m0 + row) * ldc + n0 + col;
+    if (mode == 0) {
+        Cr[o] -= vr;
+        Ci[o] -= vi;
+    } else if (mode == 1) {
+        Cr[o] = vr;
+        Ci[o] = vi;
+    } else if (mode == 3) {
+        Cr[o] = -vr;
+        Ci[o] = -vi;
+    } else {
+        Cr[o] += vr;
+        Ci[o] += vi;
+    }
+}
+
+// Whether k_cgemm_short takes the product: K = 4 kShortKW, M and N multiples
+// of 16, A's rows 16-byte aligned.
+template <typename T>
+inline bool cgemm_short_fits(int M, int N, int K, const T* Ar, const T* Ai, long lda) {
+    return K == 4 * kShortKW && M > 0 && N > 0 && M % 16 == 0 && N % 16 == 0 && lda % (16 / sizeof(T)) == 0 &&
+           reinterpret_cast<uintptr_t>(Ar) % 16 == 0 && reinterpret_cast<uintptr_t>(Ai) % 16 == 0;
+}
+
+// Short-K product on the context stream with per-launch timing; products
+// the kernel does not take go to cgemm_ctx.
+template <typename T>
+inline void cgemm_short_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
+                            const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
+    if (!cgemm_short_fits<T>(M, N, K, Ar, Ai, lda)) {
+        cgemm_ctx<T>(c, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+        return;
+    }
+    const int k = kt_mark(c, sizeof(T) == 4 ? KS_GEMM32 : KS_GEMM64, c->stream);
+    const long tiles = (long)(M / 16) * (N / 16);
+    const unsigned grid = (unsigned)(((tiles + 7) / 8) * 8);
+    hipLaunchKernelGGL(k_cgemm_short<T>, dim3(grid), dim3(256), 0, c->stream, M, N, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci,
+                       ldc, mode);
+    const double mnk = (double)M * N * K;
+    const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
+    kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);
 }

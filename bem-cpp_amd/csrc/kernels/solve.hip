@@ -65,6 +65,10 @@ __global__ __launch_bounds__(256) void k_colnorm2(int n, const double* __restric
     if (threadIdx.x == 0) out[j] = red[0];
 }
 
+// Updates of at most this many rows go to the short-K kernel (below it the
+// tiled GEMM's launches are load-latency bound at K = 256).
+constexpr int kShortRows = 4096;
+
 // Y <- U^{-1} L^{-1} Y for the factors at (Fr, Fi), Y planar n x ncol (ld ldy),
 // with the inverses of the 256 x 256 diagonal blocks at inv (L blocks, then U
 // blocks; each n x 256 per plane).  All products are matrix-core GEMMs:
@@ -79,21 +83,29 @@ void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, const void* inv, T* Yr, T* Y
     const T* Lii = Lir + (size_t)n * NB;
     const T* Uir = Lii + (size_t)n * NB;
     const T* Uii = Uir + (size_t)n * NB;
+    // diagonal-block products and updates of at most kShortRows rows: the
+    // short-K kernel (latency-bound shapes); longer updates: the tiled GEMM
+    auto update = [&](int rows, const T* ar, const T* ai, const T* br, const T* bi, T* cr, T* ci) {
+        if (rows <= kShortRows)
+            cgemm_short_ctx<T>(c, rows, ncol, NB, ar, ai, ld, br, bi, ldy, cr, ci, ldy, 0);
+        else
+            cgemm_ctx<T>(c, rows, ncol, NB, ar, ai, ld, br, bi, ldy, cr, ci, ldy, 0);
+    };
     for (int I = 0; I < nb; I++) {
         const long y = (long)I * NB * ldy, io = (long)I * NB * NB;
-        cgemm_ctx<T>(c, NB, ncol, NB, Lir + io, Lii + io, NB, Yr + y, Yi + y, ldy, Wr + y, Wi + y, ldy, 1);
+        cgemm_short_ctx<T>(c, NB, ncol, NB, Lir + io, Lii + io, NB, Yr + y, Yi + y, ldy, Wr + y, Wi + y, ldy, 1);
         const int rest = n - (I + 1) * NB;
         if (rest > 0) {
             const long a = (long)(I + 1) * NB * ld + (long)I * NB, yc = (long)(I + 1) * NB * ldy;
-            cgemm_ctx<T>(c, rest, ncol, NB, Fr + a, Fi + a, ld, Wr + y, Wi + y, ldy, Yr + yc, Yi + yc, ldy, 0);
+            update(rest, Fr + a, Fi + a, Wr + y, Wi + y, Yr + yc, Yi + yc);
         }
     }
     for (int I = nb - 1; I >= 0; I--) {
         const long y = (long)I * NB * ldy, io = (long)I * NB * NB;
-        cgemm_ctx<T>(c, NB, ncol, NB, Uir + io, Uii + io, NB, Wr + y, Wi + y, ldy, Yr + y, Yi + y, ldy, 1);
+        cgemm_short_ctx<T>(c, NB, ncol, NB, Uir + io, Uii + io, NB, Wr + y, Wi + y, ldy, Yr + y, Yi + y, ldy, 1);
         if (I > 0) {
             const long a = (long)I * NB;  // rows 0 .. I*NB-1, columns of block I
-            cgemm_ctx<T>(c, I * NB, ncol, NB, Fr + a, Fi + a, ld, Yr + y, Yi + y, ldy, Wr, Wi, ldy, 0);
+            update(I * NB, Fr + a, Fi + a, Yr + y, Yi + y, Wr, Wi);
         }
     }
 }
