@@ -560,7 +560,7 @@ void launch_laswp(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, long ld, int ca, int 
 
 // L11^{-1} of the NB x NB unit lower triangle at F (ld ldf) into I (ld NB) by
 // doubling.  The inverses of the eight 32 x 32 diagonal blocks come from one
-// LDS substitution launch; then for s = 32, 64, 128 every pair of s x s
+// launch (one wave per block, k_inv_lower32); then for s = 32, 64, 128 every pair of s x s
 // diagonal blocks is joined,
 //     [X11 0; X21 X22] with X21 = -X22 (L21 X11),
 // as two batched GEMMs (all pairs of a level in one launch each, on the 32 x 32
@@ -569,10 +569,9 @@ void launch_laswp(bem_ctx* c, hipStream_t s, T* Ar, T* Ai, long ld, int ca, int 
 // launches with short dependency chains replace one 256-step substitution.
 template <typename T>
 void invert_lower_doubling(hipStream_t s, const T* Fr, const T* Fi, long ldf, T* Ir, T* Ii, const LuWork<T>& w) {
-    constexpr int CW = TrsmCW<T>::value;
     constexpr int D = 32;
-    hipLaunchKernelGGL((k_trsm_slab<T, CW, true, D>), dim3(D / CW, NB / D), dim3(256), 0, s, D, Fr, Fi, ldf,
-                       (long)D * (ldf + 1), nullptr, nullptr, 0L, 0L, Ir, Ii, (long)NB, (long)D * (NB + 1), D);
+    hipLaunchKernelGGL(k_inv_lower32<T>, dim3(NB / D), dim3(64), 0, s, Fr, Fi, ldf, (long)D * (ldf + 1), Ir, Ii,
+                       (long)NB, (long)D * (NB + 1));
     for (int h = D; h < NB; h *= 2) {
         const int pairs = NB / (2 * h);
         const long sf = 2L * h * (ldf + 1), si = 2L * h * (NB + 1), st = (long)h * h;

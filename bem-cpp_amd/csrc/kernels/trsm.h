@@ -23,8 +23,7 @@ __device__ __forceinline__ void cdiv(T ar, T ai, T br, T bi, T& qr, T& qi) {
     qi = (ai * br - ar * bi) / d;
 }
 
-// RMAX: largest R of the launch (sizes the LDS slab; 32 for the 32 x 32 blocks
-// of the doubling inverse, so that launch fits beside a running GEMM).
+// RMAX: largest R of the launch (sizes the LDS slab).
 template <typename T, int CW, bool LOWER, int RMAX = 256>
 __global__ __launch_bounds__(256) void k_trsm_slab(int R, const T* __restrict__ Tr, const T* __restrict__ Ti,
                                                    long ldt, long bst, const T* __restrict__ Br,
@@ -114,6 +113,54 @@ __global__ __launch_bounds__(256) void k_trsm_slab(int R, const T* __restrict__ 
             Xr[(long)t * ldx + c0 + c] = xr[c];
             Xi[(long)t * ldx + c0 + c] = xi[c];
         }
+}
+
+// Inverses of unit lower triangular 32 x 32 blocks, one wave per block
+// (blockIdx.x): the block is staged once in LDS, then lane j < 32 forms
+// column j of X = L^{-1} by forward substitution in registers, with no
+// barrier inside the substitution (the slab kernel above takes one per row).
+// x_i starts at delta_ij and takes the terms k = 0 .. i-1 in that order with
+// the slab kernel's four fused multiply-adds per complex term, so the result
+// is bit for bit that of k_trsm_slab with B = I.
+template <typename T>
+__global__ __launch_bounds__(64) void k_inv_lower32(const T* __restrict__ Fr, const T* __restrict__ Fi, long ldf,
+                                                    long bsf, T* __restrict__ Xr, T* __restrict__ Xi, long ldx,
+                                                    long bsx) {
+    __builtin_amdgcn_s_setprio(3);  // on the factorisation's critical path beside the trailing GEMM
+    __shared__ T lr[32][33], li[32][33];
+    Fr += blockIdx.x * bsf;
+    Fi += blockIdx.x * bsf;
+    Xr += blockIdx.x * bsx;
+    Xi += blockIdx.x * bsx;
+    const int t = threadIdx.x;
+    for (int e = t; e < 32 * 32; e += 64) {
+        const int r = e >> 5, c = e & 31;
+        lr[r][c] = Fr[(long)r * ldf + c];
+        li[r][c] = Fi[(long)r * ldf + c];
+    }
+    __syncthreads();
+    if (t >= 32) return;
+    T xr[32], xi[32];
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        T ar = (i == t) ? T(1) : T(0), ai = T(0);
+#pragma unroll
+        for (int k = 0; k < i; k++) {
+            const T a = lr[i][k], b = li[i][k];
+            const T na = -a, nb = -b;
+            ar = fmadd(na, xr[k], ar);
+            ar = fmadd(b, xi[k], ar);
+            ai = fmadd(na, xi[k], ai);
+            ai = fmadd(nb, xr[k], ai);
+        }
+        xr[i] = ar;
+        xi[i] = ai;
+    }
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        Xr[(long)i * ldx + t] = xr[i];
+        Xi[(long)i * ldx + t] = xi[i];
+    }
 }
 
 // Column-chunk width per precision (registers per thread: 2 CW values).
