@@ -930,6 +930,7 @@ size_t asm_record_bytes(const bem_ctx* c) {
 int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq, double* extra, const double* Wr,
                         const double* Wi, long ldw, double* out) {
     if (nq < 1 || nq > kMaxRecSets) return bem_fail(c, BEM_ERR_ARG, "contraction: 1..3 coefficient sets");
+    if (nq > 1 && !extra) return bem_fail(c, BEM_ERR_ARG, "contraction: no record arrays for the extra sets");
     for (int j = 1; j < nq; j++)
         if (q[j].deriv != q[0].deriv) return bem_fail(c, BEM_ERR_ARG, "contraction: sets of one moment kind");
     const double *dverts, *dqw;
