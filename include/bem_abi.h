@@ -272,7 +272,12 @@ int bem_orient_mueller(bem_ctx* ctx, double kext_re, double kext_im, double eta_
  * for k_ext = ka, k_int = ka m, eta_ext = 1, eta_int = 1/|m|
  * (main.cpp:192-196).  Uses one adjoint solve with the existing
  * factorisation (Z^T = S Z S, S = diag(I, -I)) and a matrix-free contraction
- * of dZ/dp.  grad receives 3 doubles. */
+ * of dZ/dp.  grad receives 3 doubles.  Device memory for the call: a work
+ * area of about 10 npad x nrhs (nrhs rounded up to 16) + 2 npad^2 doubles,
+ * plus two extra triangle-pair record arrays (2 x 54 x T (T + 1) / 2
+ * doubles; 16 GB at 2N = 18432); when the record arrays do not fit, the
+ * contraction runs once per parameter instead (same result, about three
+ * times the assembly passes). */
 int bem_mueller_grad(bem_ctx* ctx, const double* dL_dM, double* grad);
 
 #ifdef __cplusplus
