@@ -731,6 +731,7 @@ int bem_solve(bem_ctx* c, const double* B, int nrhs, double* X, double ir_tol, i
     c->t_ms[2] = ts;
     c->t_ms[3] = ti;
     c->has_X = true;
+    c->x_tol = ir_max > 0 ? ir_tol : 0.0;
     if (st) {
         st->iterations = iters;
         st->residual0 = r0;
@@ -766,6 +767,7 @@ int bem_solve_gmres(bem_ctx* c, const double* B, int nrhs, double* X, double tol
     c->t_ms[2] = tms;
     c->t_ms[3] = 0.0;
     c->has_X = true;
+    c->x_tol = tol;
     if (st) {
         st->iterations = iters;
         st->residual0 = r0;
@@ -843,6 +845,7 @@ int bem_precond_apply(bem_ctx* c, int precond, const double* R, int nrhs, double
     if (rc) return rc;
     BEM_HIP(c, hipStreamSynchronize(c->stream));
     c->has_X = true;
+    c->x_tol = 0.0;
     return store_solution(c, nrhs, Zout);
 }
 

@@ -177,6 +177,7 @@ struct bem_ctx {
     double *Br = nullptr, *Bi = nullptr, *Xr = nullptr, *Xi = nullptr;
     size_t rhs_cap = 0;  // doubles per plane allocated
     bool has_B = false, has_X = false;
+    double x_tol = 0.0;  // relative residual the device solution was refined to (0: not known)
 
     // plane waves of the last bem_rhs_planewave call (k_hat, E0, H0 per RHS)
     std::vector<double> rhs_prm;

@@ -206,7 +206,10 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad) {
     hipLaunchKernelGGL(k_negate_lower, gneg, dim3(256), 0, s, lo, hi, c->Br, c->Bi);
     int iters = 0;
     double r0 = 0, r1 = 0, ts = 0, ti = 0;
-    int rc = lu_solve_launch(c, c->nrhs, 1e-12, 30, &iters, &r0, &r1, &ts, &ti);
+    // refined to the tolerance the forward solution was refined to (1e-12
+    // when that is not known): the gradient is no more accurate than X
+    const double lam_tol = c->x_tol > 1e-12 ? c->x_tol : 1e-12;
+    int rc = lu_solve_launch(c, c->nrhs, lam_tol, 30, &iters, &r0, &r1, &ts, &ti);
     if (rc < 0 && rc != BEM_ERR_NOCONV) return rc;
     BEM_HIP(c, hipMemcpyAsync(Lr, c->Xr, cnt * 8, hipMemcpyDeviceToDevice, s));
     BEM_HIP(c, hipMemcpyAsync(Li, c->Xi, cnt * 8, hipMemcpyDeviceToDevice, s));

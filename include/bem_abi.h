@@ -271,8 +271,10 @@ int bem_orient_mueller(bem_ctx* ctx, double kext_re, double kext_im, double eta_
  * call, given dL/dM_avg (4 x 4 x ntheta), with respect to (Re m, Im m, ka)
  * for k_ext = ka, k_int = ka m, eta_ext = 1, eta_int = 1/|m|
  * (main.cpp:192-196).  Uses one adjoint solve with the existing
- * factorisation (Z^T = S Z S, S = diag(I, -I)) and a matrix-free contraction
- * of dZ/dp.  grad receives 3 doubles.  Device memory for the call: a work
+ * factorisation (Z^T = S Z S, S = diag(I, -I)), refined to the relative
+ * residual the forward solution was refined to (bem_solve's ir_tol with
+ * ir_max > 0, bem_solve_gmres' tol; 1e-12 otherwise or when tighter), and a
+ * matrix-free contraction of dZ/dp.  grad receives 3 doubles.  Device memory for the call: a work
  * area of about 10 npad x nrhs (nrhs rounded up to 16) + 2 npad^2 doubles,
  * plus two extra triangle-pair record arrays (2 x 54 x T (T + 1) / 2
  * doubles; 16 GB at 2N = 18432); when the record arrays do not fit, the
