@@ -150,3 +150,29 @@ def test_mueller_grad_c4_vs_oracle_fd_fixture(bem):
         grad_G = ctx.mueller_grad(g["G"])
     for grad, fd in ((grad_log, g["fd_log_h1e6"]), (grad_G, g["fd_G_h1e6"])):
         assert np.all(np.abs(grad - fd) <= 1e-5 * np.abs(fd).max()), (grad, fd)
+
+
+@pytest.mark.gpu
+def test_mueller_grad_follows_the_forward_tolerance(oracle, bem):
+    """The adjoint solve is refined to the tolerance of the forward solve:
+    after a forward solve refined to 1e-8 the gradient stays within 1e-6 of
+    the one after a 1e-12 solve (which the FD tests above pin)."""
+    ka, m, ntheta = 1.5, 1.33 + 0.01j, 19
+    rs = np.random.RandomState(7)
+    Gw = rs.randn(4, 4, ntheta)
+    prob = oracle.Problem.from_shape("sphere", 1, 1.0)
+    grads = {}
+    with bem.Context(0) as ctx:
+        ctx.set_mesh(prob.verts, prob.tris, 7)
+        ctx.assemble(complex(ka), ka * m, 1.0, 1.0 / abs(m))
+        RT, w = bem.orientations(2, 2, 1)
+        kh, e0 = bem.planewave_orient_rhs(RT)
+        ctx.rhs_planewave(ka, 1.0, kh, e0)
+        assert ctx.factor("fp32") == 0
+        th = np.linspace(0, np.pi, ntheta)
+        for tol in (1e-12, 1e-8):
+            ctx.solve(None, ir_tol=tol, ir_max=30, return_X=False, nrhs=len(kh))
+            ctx.orient_mueller(ka, 1.0, RT, w, th)
+            grads[tol] = ctx.mueller_grad(Gw)
+    scale = np.abs(grads[1e-12]).max()
+    assert np.all(np.abs(grads[1e-8] - grads[1e-12]) <= 1e-6 * scale), grads
