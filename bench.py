@@ -133,8 +133,8 @@ def cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N_full):
         "sample": (f"FP64 oracle (oracle/bem_oracle.cpp, OpenMP {threads} threads): assembly of {shape} ref {ref_s} "
                    f"(N={p.N}) x (N/{p.N})^2, LU + {2 * n_orient}-RHS solve of a random n={n_s} system x "
                    f"(2N/{n_s})^3, far field of 2 orientations on ref {ref_s} x points ratio; extrapolated to "
-                   f"N={N_full}: assembly {t_asm:.0f} s, LU+solve {t_lu:.0f} s, far field "
-                   f"{t_ff_per_orient * n_orient:.1f} s"),
+                   f"N={N_full}: assembly {t_asm:.3g} s, LU+solve {t_lu:.3g} s, far field "
+                   f"{t_ff_per_orient * n_orient:.3g} s"),
     }
 
 
