@@ -6,7 +6,8 @@
 // (backward).  Thread t keeps row t of X (CW columns) in registers; the
 // triangle is staged through LDS in 32-column slabs with coalesced loads; the
 // finished row of each step is broadcast through a double-buffered LDS row,
-// so each step costs one barrier.  A batch of diagonal blocks runs in one
+// so each step costs one barrier (two for the upper triangle, whose row is
+// divided by CW lanes at once between them).  A batch of diagonal blocks runs in one
 // launch (blockIdx.y), and B == nullptr means B = I, which is how the
 // diagonal-block inverses used by the GEMM-based solves are formed.
 #pragma once
@@ -32,6 +33,7 @@ __global__ __launch_bounds__(256) void k_trsm_slab(int R, const T* __restrict__ 
     __builtin_amdgcn_s_setprio(3);  // on the factorisation's critical path beside the trailing GEMM
     __shared__ T sr[RMAX][33], si[RMAX][33];
     __shared__ T xr_s[2][CW], xi_s[2][CW];
+    __shared__ T ur_s[2][CW], ui_s[2][CW];  // upper: the pivot row before its division
     const int t = threadIdx.x, c0 = blockIdx.x * CW;
     Tr += blockIdx.y * bst;
     Ti += blockIdx.y * bst;
@@ -75,24 +77,42 @@ __global__ __launch_bounds__(256) void k_trsm_slab(int R, const T* __restrict__ 
         for (int k = 0; k < 32; k++) {
             const int l = LOWER ? s + k : s + 31 - k;
             const int buf = l & 1;
-            if (t == l) {
-                if (!LOWER) {
-                    const T dr = sr[l][l - s], di = si[l][l - s];
+            if (LOWER) {
+                if (t == l) {
 #pragma unroll
                     for (int c = 0; c < CW; c++) {
-                        T qr, qi;
-                        cdiv(xr[c], xi[c], dr, di, qr, qi);
-                        xr[c] = qr;
-                        xi[c] = qi;
+                        xr_s[buf][c] = xr[c];
+                        xi_s[buf][c] = xi[c];
                     }
                 }
+                __syncthreads();
+            } else {
+                // the row goes out undivided, CW lanes divide one entry each
+                // (the same division per entry as by the row's own thread),
+                // and the row's thread takes the quotients back
+                if (t == l) {
 #pragma unroll
-                for (int c = 0; c < CW; c++) {
-                    xr_s[buf][c] = xr[c];
-                    xi_s[buf][c] = xi[c];
+                    for (int c = 0; c < CW; c++) {
+                        ur_s[buf][c] = xr[c];
+                        ui_s[buf][c] = xi[c];
+                    }
+                }
+                __syncthreads();
+                if (t < CW) {
+                    T qr, qi;
+                    cdiv(ur_s[buf][t], ui_s[buf][t], sr[l][l - s], si[l][l - s], qr, qi);
+                    xr_s[buf][t] = qr;
+                    xi_s[buf][t] = qi;
+                }
+                __syncthreads();
+                if (t == l) {
+#pragma unroll
+                    for (int c = 0; c < CW; c++) {
+                        xr[c] = xr_s[buf][c];
+                        xi[c] = xi_s[buf][c];
+                    }
                 }
             }
-            __syncthreads();
             if (LOWER ? (t > l && t < R) : (t < l)) {
                 const T a = sr[t][l - s], b = si[t][l - s];
                 const T na = -a, nb = -b;
