@@ -159,8 +159,9 @@ def main():
                          "reference for the FETCH_SIZE / WRITE_SIZE counters of profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by orientation, "
-              "see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path would shard by "
+              "orientation: every GPU factors Z and solves its share of the orientations, then one all-reduce "
+              "sums the Mueller partial sums)", file=sys.stderr)
         sys.exit(2)
 
     import bem_amd
