@@ -25,6 +25,8 @@ F64 = 0
 F32_3M = 1
 ASM_Z64 = 1
 ASM_Z32 = 2
+ERR_ARG = -1
+ERR_STATE = -2
 ERR_NOCONV = -5
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -79,6 +81,7 @@ def lib():
         "bem_set_mesh": ([vp, _dp, i, _ip, i, i], i), "bem_num_rwg": ([vp], i),
         "bem_get_rwg": ([vp, _ip, _ip, _dp, _dp, _dp, _dp, _dp], i),
         "bem_assemble": ([vp, d, d, d, d, d, d, u], i), "bem_get_Z": ([vp, _dp, i], i),
+        "bem_get_Z_entries": ([vp, _ip, _ip, i, _dp], i),
         "bem_assemble_LK": ([vp, d, d, _dp, _dp], i),
         "bem_factor": ([vp, i], i), "bem_set_Z": ([vp, _dp, i, i], i), "bem_get_pivots": ([vp, _ip], i),
         "bem_get_lu": ([vp, _dp], i), "bem_set_lu": ([vp, _dp, _ip, i], i),
@@ -255,6 +258,16 @@ class Context:
         self._check(self._L.bem_get_Z(self._h, _p(Z), 1), "get_Z")
         return Z
 
+    def get_Z_entries(self, rows, cols):
+        """Entries Z[rows[e], cols[e]] of the FP64 matrix (0-based), complex128."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        cl = np.ascontiguousarray(cols, dtype=np.int32)
+        if r.shape != cl.shape:
+            raise ValueError("rows and cols differ in shape")
+        out = np.zeros(r.size, np.complex128)
+        self._check(self._L.bem_get_Z_entries(self._h, _pi(r), _pi(cl), r.size, _p(out)), "get_Z_entries")
+        return out.reshape(r.shape)
+
     def assemble_LK(self, k: complex):
         k = complex(k)
         L = np.zeros((self.N, self.N), np.complex128)
@@ -386,10 +399,14 @@ class Context:
                     "orient_mueller")
         return (M, Mo) if per_orient else M
 
-    def mueller_grad(self, dL_dM):
+    def mueller_grad(self, dL_dM, allow_noconv=False):
+        """d L / d (Re m, Im m, ka).  Raises BemError when the adjoint solve
+        misses its tolerance, unless allow_noconv (the gradient is then still
+        returned)."""
         g = np.zeros(3)
         d = _cd(dL_dM)
-        self._check(self._L.bem_mueller_grad(self._h, _p(d), _p(g)), "mueller_grad")
+        self._check(self._L.bem_mueller_grad(self._h, _p(d), _p(g)), "mueller_grad",
+                    allow=(ERR_NOCONV,) if allow_noconv else ())
         return g
 
     # GEMM configurations (block tile; "k8": K slice of 8 instead of 16)

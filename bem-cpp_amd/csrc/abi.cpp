@@ -102,11 +102,11 @@ void free_matrices(bem_ctx* c) {
     dfree(c->ipiv);
     dfree(c->perm);
     if (c->inv32) (void)hipFree(c->inv32);
-    if (c->inv64) (void)hipFree(c->inv64);
-    c->inv32 = c->inv64 = nullptr;
-    c->inv32_bytes = c->inv64_bytes = 0;
+    c->inv32 = nullptr;
+    c->inv32_bytes = 0;
     c->has_Z64 = c->has_Z32 = false;
     c->factored = -1;
+    c->lu_avail = -1;
     c->factors_stale = false;
 }
 
@@ -165,17 +165,39 @@ struct EventTimer {
 
 }  // namespace
 
-// B/X buffers: planar FP64, row-major n x ldr (ldr multiple of 16), rows padded to npad.
+// The device solutions no longer belong to the device right-hand sides (new
+// right-hand sides, or buffers reallocated or re-strided): orient_mueller and
+// mueller_grad refuse to run until the next solve (and orient_mueller).
+static void drop_solution(bem_ctx* c) {
+    c->has_X = false;
+    c->x_tol = 0.0;
+    c->last_norient = 0;
+}
+
+// B/X buffers: planar FP64, row-major n x ldr (ldr multiple of 16), rows padded
+// to npad.  Called before new right-hand sides are written: the old solutions
+// are dropped.  On an allocation failure no buffer is left half-allocated.
 int ctx_ensure_rhs(bem_ctx* c, int nrhs) {
     const int ldr = round_up(std::max(nrhs, 1), 16);
     const size_t need = (size_t)c->npad * ldr;
+    drop_solution(c);
     if (need > c->rhs_cap) {
-        if (int rc = dalloc(c, c->Br, need, "B (real)")) return rc;
-        if (int rc = dalloc(c, c->Bi, need, "B (imag)")) return rc;
-        if (int rc = dalloc(c, c->Xr, need, "X (real)")) return rc;
-        if (int rc = dalloc(c, c->Xi, need, "X (imag)")) return rc;
+        c->has_B = false;
+        c->rhs_cap = 0;
+        int rc = dalloc(c, c->Br, need, "B (real)");
+        if (!rc) rc = dalloc(c, c->Bi, need, "B (imag)");
+        if (!rc) rc = dalloc(c, c->Xr, need, "X (real)");
+        if (!rc) rc = dalloc(c, c->Xi, need, "X (imag)");
+        if (rc) {
+            dfree(c->Br);
+            dfree(c->Bi);
+            dfree(c->Xr);
+            dfree(c->Xi);
+            return rc;
+        }
         c->rhs_cap = need;
     }
+    if (ldr != c->ldr) c->has_B = false;  // the old right-hand sides have another stride
     c->nrhs = nrhs;
     c->ldr = ldr;
     return 0;
@@ -508,6 +530,7 @@ int bem_assemble(bem_ctx* c, double ker, double kei, double kir, double kii, dou
     // FP32 factors live in the FP32 working copy, which a Z32 assembly
     // overwrites; other factors stay as the preconditioner of
     // bem_solve_gmres but belong to an earlier matrix
+    if (w32 && c->lu_avail == BEM_F32_3M) c->lu_avail = -1;
     if (c->factored == BEM_F32_3M && w32)
         c->factored = -1;
     else if (c->factored >= 0)
@@ -530,6 +553,28 @@ int bem_get_Z(bem_ctx* c, double* Z, int row_major) {
             Z[2 * o] = re[(size_t)i * ld + j];
             Z[2 * o + 1] = im[(size_t)i * ld + j];
         }
+    return 0;
+}
+
+int bem_get_Z_entries(bem_ctx* c, const int* rows, const int* cols, int count, double* out) {
+    if (!c || !rows || !cols || !out || count < 0) return bem_fail(c, BEM_ERR_ARG, "get_Z_entries: bad arguments");
+    if (!c->has_Z64) return bem_fail(c, BEM_ERR_STATE, "get_Z_entries: no FP64 Z on the device");
+    for (int e = 0; e < count; e++)
+        if (rows[e] < 0 || rows[e] >= c->n || cols[e] < 0 || cols[e] >= c->n)
+            return bem_fail(c, BEM_ERR_ARG, "get_Z_entries: index out of range");
+    if (count == 0) return 0;
+    (void)hipSetDevice(c->device);
+    const size_t ib = ((size_t)count * sizeof(int) + 255) / 256 * 256;
+    void* s = nullptr;
+    if (int rc = ctx_scratch(c, 2 * ib + 2 * (size_t)count * sizeof(double), &s)) return rc;
+    int* dr = static_cast<int*>(s);
+    int* dc = reinterpret_cast<int*>(static_cast<char*>(s) + ib);
+    double* dout = reinterpret_cast<double*>(static_cast<char*>(s) + 2 * ib);
+    BEM_HIP(c, hipMemcpyAsync(dr, rows, (size_t)count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    BEM_HIP(c, hipMemcpyAsync(dc, cols, (size_t)count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (int rc = z_entries_launch(c, dr, dc, count, dout)) return rc;
+    BEM_HIP(c, hipMemcpyAsync(out, dout, 2 * (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BEM_HIP(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -605,13 +650,14 @@ int bem_factor(bem_ctx* c, int precision) {
     // the FP32 working copy now holds factors
     if (precision == BEM_F32_3M) c->has_Z32 = false;
     c->factored = info == 0 ? precision : -1;
+    c->lu_avail = precision;  // also after a zero pivot (Zgetrf's factors, kept for get_lu)
     c->factors_stale = false;
     return info;
 }
 
 int bem_get_pivots(const bem_ctx* c, int* ipiv) {
     if (!c || !ipiv) return BEM_ERR_ARG;
-    if (c->factored < 0) return BEM_ERR_STATE;
+    if (c->lu_avail < 0) return BEM_ERR_STATE;
     std::vector<int> h(c->npad);
     if (hipMemcpy(h.data(), c->ipiv, c->npad * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return BEM_ERR_HIP;
     for (int i = 0; i < c->n; i++) ipiv[i] = h[i] + 1;
@@ -620,7 +666,7 @@ int bem_get_pivots(const bem_ctx* c, int* ipiv) {
 
 int bem_get_lu(bem_ctx* c, double* LU) {
     if (!c || !LU) return BEM_ERR_ARG;
-    if (c->factored != BEM_F64) return bem_fail(c, BEM_ERR_STATE, "get_lu: no FP64 factorisation");
+    if (c->lu_avail != BEM_F64) return bem_fail(c, BEM_ERR_STATE, "get_lu: no FP64 factorisation");
     const int n = c->n, ld = c->npad;
     std::vector<double> re((size_t)n * ld), im((size_t)n * ld);
     BEM_HIP(c, hipMemcpy(re.data(), c->Vr, re.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -668,8 +714,8 @@ int bem_set_lu(bem_ctx* c, const double* LU, const int* ipiv, int n) {
     BEM_HIP(c, hipMemcpy(c->Vi, im.data(), nn * sizeof(double), hipMemcpyHostToDevice));
     BEM_HIP(c, hipMemcpy(c->ipiv, ip.data(), ld * sizeof(int), hipMemcpyHostToDevice));
     BEM_HIP(c, hipMemcpy(c->perm, perm.data(), ld * sizeof(int), hipMemcpyHostToDevice));
-    if (int rc = lu_inverses_launch(c)) return rc;
     c->factored = BEM_F64;
+    c->lu_avail = BEM_F64;
     c->factors_stale = false;
     c->has_Z64 = false;  // no system matrix: solves run without refinement
     return 0;

@@ -156,14 +156,13 @@ struct bem_ctx {
     float* Wi = nullptr;
     double* Vr = nullptr;  // FP64 LU factors
     double* Vi = nullptr;
-    void* inv32 = nullptr;  // inverses of the 256 x 256 diagonal blocks of L and U (FP32 factors)
-    size_t inv32_bytes = 0;
-    void* inv64 = nullptr;  // the same for the FP64 factors
-    size_t inv64_bytes = 0;
+    void* inv32 = nullptr;  // inverses of the 256 x 256 diagonal blocks of L and U (FP32 factors;
+    size_t inv32_bytes = 0; // the FP64 solves substitute with the diagonal blocks themselves)
     int* ipiv = nullptr;   // [npad] LAPACK-style swaps (0-based row swapped with row i)
     int* perm = nullptr;   // [npad] row permutation: row i of PZ is row perm[i] of Z
     BlockJacobi bj;        // block-Jacobi preconditioner (bem_block_jacobi)
-    int factored = -1;     // -1 none, BEM_F64, BEM_F32_3M
+    int factored = -1;     // -1 none, BEM_F64, BEM_F32_3M (factors usable for solves)
+    int lu_avail = -1;     // precision of the last factorisation, also one that met a zero pivot (get_lu)
     bool factors_stale = false;  // the factors belong to an earlier matrix (preconditioner only)
     AsmParams asm_params{};
     bool asm_valid = false;
@@ -214,7 +213,6 @@ int ctx_scratch(bem_ctx* c, size_t bytes, void** out);
 // Launchers (kernels/*.hip).  All enqueue on c->stream and return 0 or < 0.
 int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32);
 int lu_factor_launch(bem_ctx* c, int precision);
-int lu_inverses_launch(bem_ctx* c);  // FP64 diagonal-block inverses of factors in Vr/Vi
 // D = (P^T L U)^{-1} S with the context's factors (FP32 or FP64), S and D
 // planar FP64 npad x ldr (D must not alias S); work holds
 // lu_precond_work_bytes(c) bytes.  Enqueued on the context stream.
@@ -274,6 +272,9 @@ int ff_adjoint_launch(bem_ctx* c, const double* dL_dM, double* gr, double* gi, d
 // own: `extra` holds (nq - 1) * asm_record_bytes(c) bytes.
 int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq, double* extra, const double* Wr,
                         const double* Wi, long ldw, double* out);
+// Gathers count entries Z64[rows[e]][cols[e]] (device index arrays, checked
+// by the caller) into out (device, interleaved complex).
+int z_entries_launch(bem_ctx* c, const int* rows, const int* cols, int count, double* out);
 // Bytes of one record array (T (T + 1) / 2 triangle-pair records).
 size_t asm_record_bytes(const bem_ctx* c);
 

@@ -83,31 +83,38 @@ void assemble_pmchwt(const RWG& rwg, const Mesh& mesh, cdouble k_ext, cdouble k_
     }
 }
 
+// solver.cu:7-48: the factors and pivots are copied back whatever the info
+// (a zero pivot leaves a singular U, as Zgetrf does), and info is returned.
 int lu_factorize_cuda(cdouble* Z, int n, int* ipiv) {
+    if (n == 0) return 0;
     bem_ctx* c = ctx();
     g_mesh_key.clear();
     check(bem_set_Z(c, reinterpret_cast<const double*>(Z), n, 1), "lu_factorize");
     const int info = bem_factor(c, BEM_F64);
     check(info, "lu_factorize");
-    if (info > 0) {
-        std::fprintf(stderr, "  LU factorization failed: info=%d\n", info);
-        return info;
-    }
+    if (info > 0) std::fprintf(stderr, "  LU factorization failed: info=%d\n", info);
     check(bem_get_lu(c, reinterpret_cast<double*>(Z)), "lu_factorize");
     check(bem_get_pivots(c, ipiv), "lu_factorize");
-    return 0;
+    return info;
 }
 
+// solver.cu:51-84: returns the solve's info, which (as Zgetrs's) is -i for an
+// illegal i-th argument (n: 2, nrhs: 3) and 0 otherwise.
 int lu_solve_cuda(const cdouble* Z, const int* ipiv, int n, cdouble* B, int nrhs) {
+    if (n < 0) return -2;
+    if (nrhs < 0) return -3;
+    if (n == 0 || nrhs == 0) return 0;
     bem_ctx* c = ctx();
     g_mesh_key.clear();
     check(bem_set_lu(c, reinterpret_cast<const double*>(Z), ipiv, n), "lu_solve");
-    check(bem_solve(c, reinterpret_cast<const double*>(B), nrhs, reinterpret_cast<double*>(B), 0.0, 0, nullptr),
-          "lu_solve");
-    return 0;
+    const int rc =
+        bem_solve(c, reinterpret_cast<const double*>(B), nrhs, reinterpret_cast<double*>(B), 0.0, 0, nullptr);
+    check(rc, "lu_solve");
+    return rc;
 }
 
 int lu_solve_full(cdouble* Z, int n, cdouble* B, int nrhs) {
+    if (n == 0) return 0;
     bem_ctx* c = ctx();
     g_mesh_key.clear();
     check(bem_set_Z(c, reinterpret_cast<const double*>(Z), n, 1), "lu_solve_full");

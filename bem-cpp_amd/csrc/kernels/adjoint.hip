@@ -16,6 +16,8 @@
 // prefactors and the 1/|k|^2 normalisation).
 #include <cmath>
 
+#include <cstdlib>
+
 #include "../bem_internal.h"
 #include "gemm.h"
 
@@ -195,6 +197,32 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad) {
     BEM_HIP(c, hipMemcpyAsync(Bsi, c->Bi, cnt * 8, hipMemcpyDeviceToDevice, s));
     BEM_HIP(c, hipMemcpyAsync(Xsr, c->Xr, cnt * 8, hipMemcpyDeviceToDevice, s));
     BEM_HIP(c, hipMemcpyAsync(Xsi, c->Xi, cnt * 8, hipMemcpyDeviceToDevice, s));
+    // B and X are overwritten by the adjoint solve below; every exit from here
+    // on puts the forward right-hand sides and solutions back (or, if even
+    // the copies fail, marks them as gone)
+    struct Restore {
+        bem_ctx* c;
+        const double *br, *bi, *xr, *xi;
+        long cnt;
+        bool done = false;
+        int run() {
+            if (done) return 0;
+            done = true;
+            hipError_t e = hipMemcpyAsync(c->Br, br, cnt * 8, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(c->Bi, bi, cnt * 8, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(c->Xr, xr, cnt * 8, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(c->Xi, xi, cnt * 8, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // before the save buffers are freed
+            if (e != hipSuccess) {
+                c->has_B = c->has_X = false;
+                c->last_norient = 0;
+                return bem_fail(c, BEM_ERR_HIP, std::string("mueller_grad: restoring B and X: ") +
+                                                    hipGetErrorString(e));
+            }
+            return 0;
+        }
+        ~Restore() { (void)run(); }
+    } restore{c, Bsr, Bsi, Xsr, Xsi, cnt};
 
     // 1. g = dL/dX into B (far-field adjoint), explicit dL/dka
     double expl_k = 0.0;
@@ -208,17 +236,15 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad) {
     double r0 = 0, r1 = 0, ts = 0, ti = 0;
     // refined to the tolerance the forward solution was refined to (1e-12
     // when that is not known): the gradient is no more accurate than X
-    const double lam_tol = c->x_tol > 1e-12 ? c->x_tol : 1e-12;
-    int rc = lu_solve_launch(c, c->nrhs, lam_tol, 30, &iters, &r0, &r1, &ts, &ti);
-    if (rc < 0 && rc != BEM_ERR_NOCONV) return rc;
+    double lam_tol = c->x_tol > 1e-12 ? c->x_tol : 1e-12;
+    if (const char* e = std::getenv("BEM_ADJOINT_TOL")) lam_tol = std::atof(e);
+    const int solve_rc = lu_solve_launch(c, c->nrhs, lam_tol, 30, &iters, &r0, &r1, &ts, &ti);
+    if (solve_rc < 0 && solve_rc != BEM_ERR_NOCONV) return solve_rc;
     BEM_HIP(c, hipMemcpyAsync(Lr, c->Xr, cnt * 8, hipMemcpyDeviceToDevice, s));
     BEM_HIP(c, hipMemcpyAsync(Li, c->Xi, cnt * 8, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_negate_lower, gneg, dim3(256), 0, s, lo, hi, Lr, Li);
     // restore the forward state
-    BEM_HIP(c, hipMemcpyAsync(c->Br, Bsr, cnt * 8, hipMemcpyDeviceToDevice, s));
-    BEM_HIP(c, hipMemcpyAsync(c->Bi, Bsi, cnt * 8, hipMemcpyDeviceToDevice, s));
-    BEM_HIP(c, hipMemcpyAsync(c->Xr, Xsr, cnt * 8, hipMemcpyDeviceToDevice, s));
-    BEM_HIP(c, hipMemcpyAsync(c->Xi, Xsi, cnt * 8, hipMemcpyDeviceToDevice, s));
+    if (int e = restore.run()) return e;
 
     // 3. W = Lambda X^T (npad x npad, K = ldr, a multiple of 16)
     hipLaunchKernelGGL(k_transpose, dim3((ldr + 31) / 32, (npad + 31) / 32), dim3(256), 0, s, npad, ldr, Xsr, Xsi,
@@ -267,5 +293,9 @@ int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad) {
     for (int p = 0; p < 3; p++) grad[p] = -2.0 * contr[p].r;
     grad[2] += 2.0 * rhs_term.r + expl_k;
     BEM_HIP(c, hipGetLastError());
+    if (solve_rc == BEM_ERR_NOCONV)  // the gradient is written, but lambda missed its tolerance
+        return bem_fail(c, BEM_ERR_NOCONV,
+                        "mueller_grad: the adjoint solve reached a relative residual of " + std::to_string(r1) +
+                            ", above its tolerance " + std::to_string(lam_tol) + " (gradient written)");
     return 0;
 }

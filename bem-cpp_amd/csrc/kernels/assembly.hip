@@ -922,6 +922,26 @@ int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
     return 0;
 }
 
+namespace {
+// out[2e], out[2e+1] = Z64[rows[e]][cols[e]] (planar, ld = npad)
+__global__ __launch_bounds__(256) void k_z_entries(int count, const int* __restrict__ rows,
+                                                   const int* __restrict__ cols, const double* __restrict__ Zr,
+                                                   const double* __restrict__ Zi, long ld, double* __restrict__ out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= count) return;
+    const long o = (long)rows[e] * ld + cols[e];
+    out[2 * e] = Zr[o];
+    out[2 * e + 1] = Zi[o];
+}
+}  // namespace
+
+int z_entries_launch(bem_ctx* c, const int* rows, const int* cols, int count, double* out) {
+    hipLaunchKernelGGL(k_z_entries, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, count, rows, cols,
+                       c->Zr, c->Zi, (long)c->npad, out);
+    BEM_HIP(c, hipGetLastError());
+    return 0;
+}
+
 size_t asm_record_bytes(const bem_ctx* c) {
     const size_t T = (size_t)c->dm.T;
     return T * (T + 1) / 2 * kRec * sizeof(double);

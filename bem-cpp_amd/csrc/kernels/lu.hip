@@ -635,7 +635,9 @@ void factor_panel(bem_ctx* c, hipStream_t s, hipStream_t ss, T* Ar, T* Ai, int k
 }
 
 // U12 = L11^{-1} A12 and A22 -= L21 U12 for the columns [c0, c0 + nc) right
-// of outer panel k0, on stream us; U12 goes through the work buffer.
+// of outer panel k0, on stream us; U12 goes through the work buffer.  FP32:
+// U12 as a GEMM with the explicit L11^{-1} (Lir, Lii); FP64: by substitution
+// with L11 itself (k_trsm_rhs), the backward-stable form of Zgetrf's TRSM.
 template <typename T>
 void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, const T* Lii, int k0, int c0, int nc,
                     const LuWork<T>& w, bool headroom) {
@@ -646,7 +648,14 @@ void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, cons
     const long o12 = (long)k0 * ld + c0, o21 = (long)(k0 + NB) * ld + k0, o22 = (long)(k0 + NB) * ld + c0;
     T* ubr = w.ubr + (c0 - (k0 + NB));
     T* ubi = w.ubi + (c0 - (k0 + NB));
-    cgemm_ctx<T>(c, NB, nc, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1, us);
+    if constexpr (sizeof(T) == 8) {
+        const long o11 = (long)k0 * ld + k0;
+        const int kt = kt_mark(c, KS_TRSM, us);
+        trsm_rhs<T>(us, true, Ar + o11, Ai + o11, ld, Ar + o12, Ai + o12, ld, ubr, ubi, ld, nc);
+        kt_close(c, kt, 4.0 * (double)NB * NB * nc, 2.0 * sizeof(T) * ((double)NB * NB + 2.0 * NB * nc));
+    } else {
+        cgemm_ctx<T>(c, NB, nc, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1, us);
+    }
     cgemm_ctx<T>(c, nr, nc, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0, us,
                  GEMM_UPDATE | (headroom ? GEMM_HEADROOM : 0));
     const int kt = kt_mark(c, KS_OTHER, us);
@@ -682,9 +691,11 @@ void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWo
         const long o11 = (long)k0 * ld + k0;
         const long io = (long)K * NB * NB;
         (void)hipEventRecord(e_panel, ps);
-        int kt = kt_mark(c, KS_TRSM, ps);
-        invert_lower_doubling<T>(ps, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, w);
-        kt_close(c, kt, 8.0 * NB * NB * NB / 6.0, 4.0 * sizeof(T) * (double)NB * NB);
+        if constexpr (sizeof(T) == 4) {  // FP64 solves with L11 by substitution instead
+            const int kt = kt_mark(c, KS_TRSM, ps);
+            invert_lower_doubling<T>(ps, Ar + o11, Ai + o11, ld, Lir + io, Lii + io, w);
+            kt_close(c, kt, 8.0 * NB * NB * NB / 6.0, 4.0 * sizeof(T) * (double)NB * NB);
+        }
         (void)hipEventRecord(e_linv, ps);
         (void)hipStreamWaitEvent(us, e_panel, 0);
         if (nr <= 0) {
@@ -703,10 +714,13 @@ void factor(bem_ctx* c, T* Ar, T* Ai, T* Lir, T* Lii, T* Uir, T* Uii, const LuWo
         if (nr > NB)
             update_columns<T>(c, us, Ar, Ai, Lir, Lii, k0, k0 + 2 * NB, nr - NB, w, ps != us && c->gemm_plan[sizeof(T) == 4 ? 0 : 1].cap > 0);
     }
-    // U^{-1} of all diagonal blocks in one launch
-    const int kt = kt_mark(c, KS_TRSM, us);
-    invert_diag_blocks<T>(us, NB, nb, Ar, Ai, ld, Uir, Uii, false);
-    kt_close(c, kt, 8.0 * (double)n * NB * NB / 6.0, 4.0 * sizeof(T) * (double)n * NB);
+    // U^{-1} of all diagonal blocks in one launch (FP32; the FP64 solves
+    // substitute with the diagonal blocks themselves)
+    if constexpr (sizeof(T) == 4) {
+        const int kt = kt_mark(c, KS_TRSM, us);
+        invert_diag_blocks<T>(us, NB, nb, Ar, Ai, ld, Uir, Uii, false);
+        kt_close(c, kt, 8.0 * (double)n * NB * NB / 6.0, 4.0 * sizeof(T) * (double)n * NB);
+    }
     if (us != c->stream) {  // the context stream continues after the last update
         (void)hipEventRecord(e_start, us);
         (void)hipStreamWaitEvent(c->stream, e_start, 0);
@@ -722,19 +736,20 @@ __global__ void k_convert(const double* __restrict__ a, T* __restrict__ b, size_
 template <typename T>
 int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     const int n = c->npad;
-    // diagonal-block inverses: L and U, npad x NB per plane each
+    // diagonal-block inverses (FP32 only): L and U, npad x NB per plane each
+    constexpr bool with_inv = sizeof(T) == 4;
     const size_t ib = 4 * (size_t)n * NB * sizeof(T);
-    if (inv_bytes < ib) {
+    if (with_inv && inv_bytes < ib) {
         if (inv) (void)hipFree(inv);
         inv = nullptr;
         inv_bytes = 0;
         if (hipMalloc(&inv, ib) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "LU: diagonal-block inverses");
         inv_bytes = ib;
     }
-    T* Lir = static_cast<T*>(inv);
-    T* Lii = Lir + (size_t)n * NB;
-    T* Uir = Lii + (size_t)n * NB;
-    T* Uii = Uir + (size_t)n * NB;
+    T* Lir = with_inv ? static_cast<T*>(inv) : nullptr;
+    T* Lii = with_inv ? Lir + (size_t)n * NB : nullptr;
+    T* Uir = with_inv ? Lii + (size_t)n * NB : nullptr;
+    T* Uii = with_inv ? Uir + (size_t)n * NB : nullptr;
     // work: info, counters, row-pair list, candidate ids, candidate values,
     // diagonal block, U12 buffer
     const int nsets0 = (n + Tslu<T>::CH - 1) / Tslu<T>::CH;
@@ -762,34 +777,12 @@ int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     w.ti = w.tr + (size_t)(NB / 2) * (NB / 2);
     BEM_HIP(c, hipMemsetAsync(w.info, 0, (64 + ncnt) * sizeof(int), c->stream));
     // the doubling writes only the diagonal and lower blocks of each L11^{-1}
-    BEM_HIP(c, hipMemsetAsync(Lir, 0, 2 * (size_t)n * NB * sizeof(T), c->stream));
+    if (with_inv) BEM_HIP(c, hipMemsetAsync(Lir, 0, 2 * (size_t)n * NB * sizeof(T), c->stream));
     factor<T>(c, Ar, Ai, Lir, Lii, Uir, Uii, w);
     return 0;
 }
 
 }  // namespace
-
-// Diagonal-block inverses for FP64 factors loaded from the host (bem_set_lu).
-int lu_inverses_launch(bem_ctx* c) {
-    const int n = c->npad;
-    const size_t ib = 4 * (size_t)n * NB * sizeof(double);
-    if (c->inv64_bytes < ib) {
-        if (c->inv64) (void)hipFree(c->inv64);
-        c->inv64 = nullptr;
-        c->inv64_bytes = 0;
-        if (hipMalloc(&c->inv64, ib) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "LU: diagonal-block inverses");
-        c->inv64_bytes = ib;
-    }
-    double* Lir = static_cast<double*>(c->inv64);
-    double* Lii = Lir + (size_t)n * NB;
-    double* Uir = Lii + (size_t)n * NB;
-    double* Uii = Uir + (size_t)n * NB;
-    invert_diag_blocks<double>(c->stream, NB, n / NB, c->Vr, c->Vi, n, Lir, Lii, true);
-    invert_diag_blocks<double>(c->stream, NB, n / NB, c->Vr, c->Vi, n, Uir, Uii, false);
-    BEM_HIP(c, hipGetLastError());
-    BEM_HIP(c, hipStreamSynchronize(c->stream));
-    return 0;
-}
 
 int lu_setup_streams(bem_ctx* c) {
     if (c->lu_streams >= 0) return 0;
@@ -886,9 +879,11 @@ int lu_factor_launch(bem_ctx* c, int precision) {
     if ((rc = lu_setup_streams(c))) return rc;
     bool trials = false;
     if (precision == BEM_F64) {
-        if ((rc = factor_trials<double>(c, c->Vr, c->Vi, c->inv64, c->inv64_bytes, &trials))) return rc;
+        void* no_inv = nullptr;  // FP64 factors keep no diagonal-block inverses
+        size_t no_inv_bytes = 0;
+        if ((rc = factor_trials<double>(c, c->Vr, c->Vi, no_inv, no_inv_bytes, &trials))) return rc;
         if ((rc = load_working_copy<double>(c, c->Vr, c->Vi))) return rc;
-        rc = factor_prec<double>(c, c->Vr, c->Vi, c->inv64, c->inv64_bytes);
+        rc = factor_prec<double>(c, c->Vr, c->Vi, no_inv, no_inv_bytes);
     } else {
         if ((rc = factor_trials<float>(c, c->Wr, c->Wi, c->inv32, c->inv32_bytes, &trials))) return rc;
         if (!c->has_Z32 || trials)

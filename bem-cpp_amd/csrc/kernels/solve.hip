@@ -69,20 +69,24 @@ __global__ __launch_bounds__(256) void k_colnorm2(int n, const double* __restric
 // tiled GEMM's launches are load-latency bound at K = 256).
 constexpr int kShortRows = 4096;
 
-// Y <- U^{-1} L^{-1} Y for the factors at (Fr, Fi), Y planar n x ncol (ld ldy),
-// with the inverses of the 256 x 256 diagonal blocks at inv (L blocks, then U
-// blocks; each n x 256 per plane).  All products are matrix-core GEMMs:
-//   forward   W_I = Linv_I Y_I;   Y_{>I} -= L_{>I,I} W_I
-//   backward  Y_I = Uinv_I W_I;   W_{<I} -= U_{<I,I} Y_I
+// Y <- U^{-1} L^{-1} Y for the factors at (Fr, Fi), Y planar n x ncol (ld ldy).
+//   forward   W_I = L_II^{-1} Y_I;   Y_{>I} -= L_{>I,I} W_I
+//   backward  Y_I = U_II^{-1} W_I;   W_{<I} -= U_{<I,I} Y_I
+// The updates are matrix-core GEMMs.  The diagonal blocks: FP32 factors apply
+// the explicit inverses of the 256 x 256 diagonal blocks at inv (L blocks,
+// then U blocks; each n x 256 per plane) as GEMMs, which refinement corrects;
+// FP64 factors are solved by substitution (k_trsm_rhs), which keeps the
+// unrefined FP64 solve backward stable as Zgetrs is (inv unused).
 // W is a work buffer of Y's shape; the result is left in Y.
 template <typename T>
 void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, const void* inv, T* Yr, T* Yi, T* Wr, T* Wi, int ncol, long ldy) {
     const int n = c->npad, nb = n / NB;
     const long ld = n;
+    constexpr bool subst = sizeof(T) == 8;
     const T* Lir = static_cast<const T*>(inv);
-    const T* Lii = Lir + (size_t)n * NB;
-    const T* Uir = Lii + (size_t)n * NB;
-    const T* Uii = Uir + (size_t)n * NB;
+    const T* Lii = subst ? nullptr : Lir + (size_t)n * NB;
+    const T* Uir = subst ? nullptr : Lii + (size_t)n * NB;
+    const T* Uii = subst ? nullptr : Uir + (size_t)n * NB;
     // diagonal-block products and updates of at most kShortRows rows: the
     // short-K kernel (latency-bound shapes); longer updates: the tiled GEMM
     auto update = [&](int rows, const T* ar, const T* ai, const T* br, const T* bi, T* cr, T* ci) {
@@ -92,8 +96,11 @@ void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, const void* inv, T* Yr, T* Y
             cgemm_ctx<T>(c, rows, ncol, NB, ar, ai, ld, br, bi, ldy, cr, ci, ldy, 0);
     };
     for (int I = 0; I < nb; I++) {
-        const long y = (long)I * NB * ldy, io = (long)I * NB * NB;
-        cgemm_short_ctx<T>(c, NB, ncol, NB, Lir + io, Lii + io, NB, Yr + y, Yi + y, ldy, Wr + y, Wi + y, ldy, 1);
+        const long y = (long)I * NB * ldy, io = (long)I * NB * NB, dg = (long)I * NB * (ld + 1);
+        if constexpr (subst)
+            trsm_rhs<T>(c->stream, true, Fr + dg, Fi + dg, ld, Yr + y, Yi + y, ldy, Wr + y, Wi + y, ldy, ncol);
+        else
+            cgemm_short_ctx<T>(c, NB, ncol, NB, Lir + io, Lii + io, NB, Yr + y, Yi + y, ldy, Wr + y, Wi + y, ldy, 1);
         const int rest = n - (I + 1) * NB;
         if (rest > 0) {
             const long a = (long)(I + 1) * NB * ld + (long)I * NB, yc = (long)(I + 1) * NB * ldy;
@@ -101,8 +108,11 @@ void lu_apply(bem_ctx* c, const T* Fr, const T* Fi, const void* inv, T* Yr, T* Y
         }
     }
     for (int I = nb - 1; I >= 0; I--) {
-        const long y = (long)I * NB * ldy, io = (long)I * NB * NB;
-        cgemm_short_ctx<T>(c, NB, ncol, NB, Uir + io, Uii + io, NB, Wr + y, Wi + y, ldy, Yr + y, Yi + y, ldy, 1);
+        const long y = (long)I * NB * ldy, io = (long)I * NB * NB, dg = (long)I * NB * (ld + 1);
+        if constexpr (subst)
+            trsm_rhs<T>(c->stream, false, Fr + dg, Fi + dg, ld, Wr + y, Wi + y, ldy, Yr + y, Yi + y, ldy, ncol);
+        else
+            cgemm_short_ctx<T>(c, NB, ncol, NB, Uir + io, Uii + io, NB, Wr + y, Wi + y, ldy, Yr + y, Yi + y, ldy, 1);
         if (I > 0) {
             const long a = (long)I * NB;  // rows 0 .. I*NB-1, columns of block I
             update(I * NB, Fr + a, Fi + a, Yr + y, Yi + y, Wr, Wi);
@@ -147,7 +157,7 @@ int lu_precond_apply(bem_ctx* c, const double* Sr, const double* Si, double* Dr,
         double* Qi = Qr + cnt;
         hipLaunchKernelGGL(k_gather_rows<double>, g1, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ncol, Yr, Yi,
                            (long)ncol);
-        lu_apply<double>(c, c->Vr, c->Vi, c->inv64, Yr, Yi, Qr, Qi, ncol, ncol);
+        lu_apply<double>(c, c->Vr, c->Vi, nullptr, Yr, Yi, Qr, Qi, ncol, ncol);
         hipLaunchKernelGGL(k_accumulate<double>, g1, dim3(256), 0, s, cnt, Yr, Yi, Dr, Di, 1);
     } else {
         return bem_fail(c, BEM_ERR_STATE, "preconditioner: no factorisation");
@@ -198,7 +208,7 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
             double* Wi = Wr + cnt;
             hipLaunchKernelGGL(k_gather_rows<double>, gr, dim3(256), 0, s, n, ncol, c->perm, Sr, Si, (long)ldr, Yr,
                                Yi, (long)ldr);
-            lu_apply<double>(c, c->Vr, c->Vi, c->inv64, Yr, Yi, Wr, Wi, ncol, ldr);
+            lu_apply<double>(c, c->Vr, c->Vi, nullptr, Yr, Yi, Wr, Wi, ncol, ldr);
             hipLaunchKernelGGL(k_accumulate<double>, g1, dim3(256), 0, s, cnt, Yr, Yi, c->Xr, c->Xi, assign);
         }
     };

@@ -183,6 +183,142 @@ __global__ __launch_bounds__(64) void k_inv_lower32(const T* __restrict__ Fr, co
     }
 }
 
+// Solves T X = B by substitution for one 256 x 256 triangular block T (unit
+// lower: forward; non-unit upper: backward) and the columns of B, CW columns
+// per block (blockIdx.x; ncols a multiple of CW).  This is the backward-
+// stable form of the FP64 solves and of U12 = L11^{-1} A12 in the FP64
+// factorisation (no explicit inverse is formed).
+//   Thread t keeps row t of X in registers.  The triangle is staged in
+// 32-column slabs.  Inside a slab the 32 rows are solved by the one wave that
+// holds them, with the finished row of each step passed through LDS inside
+// the wave (LDS operations of one wave complete in issue order, so the step
+// needs no workgroup barrier); then a barrier publishes the slab's 32 rows and
+// every row outside the slab subtracts their contribution.  Slabs are 16
+// columns wide in FP64 and 32 in FP32 (about 68 KB of LDS either way, so the
+// kernel fits on a CU beside a running trailing update).  Three barriers per
+// slab instead of one or two per row.  In the upper case the
+// pivot row is divided by CW lanes at once, one entry each.  B and X may be
+// the same array (each block reads its columns before it writes them).
+template <typename T, int CW, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm_rhs(const T* __restrict__ Tr, const T* __restrict__ Ti, long ldt,
+                                                  const T* Br, const T* Bi, long ldb, T* Xr, T* Xi, long ldx) {
+    __builtin_amdgcn_s_setprio(3);  // U12 of the next panel is on the factorisation's critical path
+    constexpr int R = 256, S = sizeof(T) == 8 ? 16 : 32;
+    __shared__ T sr[R][S + 1], si[R][S + 1];
+    __shared__ T pr[S][CW], pi[S][CW];
+    const int t = threadIdx.x, c0 = blockIdx.x * CW;
+    T xr[CW], xi[CW];
+#pragma unroll
+    for (int c = 0; c < CW; c++) {
+        xr[c] = Br[(long)t * ldb + c0 + c];
+        xi[c] = Bi[(long)t * ldb + c0 + c];
+    }
+#pragma unroll 1
+    for (int sI = 0; sI < R / S; sI++) {
+        const int s = LOWER ? sI * S : R - S * (sI + 1);
+        // rows of this slab's columns that are read: below the slab's top row
+        // (lower) or above its bottom row (upper)
+        const int r0 = LOWER ? s : 0, r1 = LOWER ? R : s + S;
+        __syncthreads();  // the previous slab's readers are done with sr / pr
+        for (int e = r0 * S + t; e < r1 * S; e += 256) {
+            const int r = e / S, cc = e % S;
+            sr[r][cc] = Tr[(long)r * ldt + s + cc];
+            si[r][cc] = Ti[(long)r * ldt + s + cc];
+        }
+        __syncthreads();
+        if ((t >> 6) == (s >> 6)) {  // the wave that holds rows s .. s + S - 1
+            const int lane0 = t & ~63;
+#pragma unroll 1
+            for (int k = 0; k < S; k++) {
+                const int l = LOWER ? s + k : s + S - 1 - k, q = l - s;
+                if (t == l) {
+#pragma unroll
+                    for (int c = 0; c < CW; c++) {
+                        pr[q][c] = xr[c];
+                        pi[q][c] = xi[c];
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (!LOWER) {
+                    if (t - lane0 < CW) {
+                        const int c = t - lane0;
+                        T qr, qi;
+                        cdiv(pr[q][c], pi[q][c], sr[l][q], si[l][q], qr, qi);
+                        pr[q][c] = qr;
+                        pi[q][c] = qi;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (t == l) {
+#pragma unroll
+                        for (int c = 0; c < CW; c++) {
+                            xr[c] = pr[q][c];
+                            xi[c] = pi[q][c];
+                        }
+                    }
+                }
+                if (LOWER ? (t > l && t < s + S) : (t >= s && t < l)) {
+                    const T a = sr[t][q], b = si[t][q];
+                    const T na = -a, nb = -b;
+#pragma unroll
+                    for (int c = 0; c < CW; c++) {
+                        const T ur = pr[q][c], ui = pi[q][c];
+                        xr[c] = fmadd(na, ur, xr[c]);
+                        xr[c] = fmadd(b, ui, xr[c]);
+                        xi[c] = fmadd(na, ui, xi[c]);
+                        xi[c] = fmadd(nb, ur, xi[c]);
+                    }
+                }
+                // the next step's row goes to another LDS row, but the
+                // division of the upper case rewrites this one: keep the
+                // reads of this step ahead of the next step's writes
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();  // pr holds the slab's S solved rows
+        if (LOWER ? (t >= s + S) : (t < s)) {
+#pragma unroll 4
+            for (int q = 0; q < S; q++) {
+                const T a = sr[t][q], b = si[t][q];
+                const T na = -a, nb = -b;
+#pragma unroll
+                for (int c = 0; c < CW; c++) {
+                    const T ur = pr[q][c], ui = pi[q][c];
+                    xr[c] = fmadd(na, ur, xr[c]);
+                    xr[c] = fmadd(b, ui, xr[c]);
+                    xi[c] = fmadd(na, ui, xi[c]);
+                    xi[c] = fmadd(nb, ur, xi[c]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CW; c++) {
+        Xr[(long)t * ldx + c0 + c] = xr[c];
+        Xi[(long)t * ldx + c0 + c] = xi[c];
+    }
+}
+
+// Column-chunk width of k_trsm_rhs (FP64: 16 registers of X per thread).
+constexpr int kTrsmRhsCW = 8;
+
+// T X = B for the 256 x 256 triangle at (Tr, Ti) and ncols columns (a
+// multiple of kTrsmRhsCW) on stream s.
+template <typename T>
+inline void trsm_rhs(hipStream_t s, bool lower, const T* Tr, const T* Ti, long ldt, const T* Br, const T* Bi,
+                     long ldb, T* Xr, T* Xi, long ldx, int ncols) {
+    const dim3 grid((unsigned)(ncols / kTrsmRhsCW));
+    if (lower)
+        hipLaunchKernelGGL((k_trsm_rhs<T, kTrsmRhsCW, true>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr, Xi,
+                           ldx);
+    else
+        hipLaunchKernelGGL((k_trsm_rhs<T, kTrsmRhsCW, false>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr,
+                           Xi, ldx);
+}
+
 // Column-chunk width per precision (registers per thread: 2 CW values).
 template <typename T>
 struct TrsmCW;

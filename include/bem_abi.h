@@ -169,6 +169,10 @@ int bem_assemble(bem_ctx* ctx, double kext_re, double kext_im, double kint_re, d
 /* Copies Z (2N x 2N, FP64) to the host, row-major (row_major=1, the reference
  * layout of pmchwt.cu:43-49) or column-major. */
 int bem_get_Z(bem_ctx* ctx, double* Z, int row_major);
+/* count entries Z[rows[e]][cols[e]] of the FP64 matrix (0-based indices) into
+ * out[2e], out[2e+1] (interleaved complex).  No reference counterpart: the
+ * sampled form of bem_get_Z, for checks of large systems without copying Z. */
+int bem_get_Z_entries(bem_ctx* ctx, const int* rows, const int* cols, int count, double* out);
 /* Replaces assemble_L_K_cuda (assembly.h:11-13): L and K (N x N row-major) for
  * one wavenumber.  Runs the same kernels with k_ext = k_int = k, eta = 1. */
 int bem_assemble_LK(bem_ctx* ctx, double k_re, double k_im, double* L, double* K);
@@ -181,11 +185,13 @@ int bem_factor(bem_ctx* ctx, int precision);
  * as Z, replacing an assembled one; used by the lu_* drop-ins. */
 int bem_set_Z(bem_ctx* ctx, const double* Z, int n, int row_major);
 /* Copies the 1-based LAPACK-style pivot vector of the last factorisation
- * (ipiv of lu_factorize_cuda, solver.h:11) into ipiv[n]. */
+ * (ipiv of lu_factorize_cuda, solver.h:11) into ipiv[n], also after one that
+ * returned info > 0 (solver.cu:39-41 copies them back whatever the info). */
 int bem_get_pivots(const bem_ctx* ctx, int* ipiv);
 /* The factors of the last FP64 factorisation, P Z = L U with unit L, as one
  * n x n interleaved complex matrix (L below, U on and above the diagonal) in
- * row-major order: what lu_factorize_cuda leaves in Z (solver.h:11). */
+ * row-major order: what lu_factorize_cuda leaves in Z (solver.h:11), also
+ * after a zero pivot (info > 0: U is singular, as Zgetrf leaves it). */
 int bem_get_lu(bem_ctx* ctx, double* LU);
 /* Loads factors and 1-based pivots in the bem_get_lu/bem_get_pivots layout as
  * the current FP64 factorisation: the (Z, ipiv) input of lu_solve_cuda
@@ -279,7 +285,11 @@ int bem_orient_mueller(bem_ctx* ctx, double kext_re, double kext_im, double eta_
  * plus two extra triangle-pair record arrays (2 x 54 x T (T + 1) / 2
  * doubles; 16 GB at 2N = 18432); when the record arrays do not fit, the
  * contraction runs once per parameter instead (same result, about three
- * times the assembly passes). */
+ * times the assembly passes).  Returns BEM_ERR_NOCONV, with grad written, when
+ * the adjoint solve misses its tolerance within 30 refinement sweeps (the
+ * environment variable BEM_ADJOINT_TOL, when set, replaces that tolerance).
+ * The device right-hand sides and solutions are those of the forward solve
+ * again when the call returns, whatever it returns. */
 int bem_mueller_grad(bem_ctx* ctx, const double* dL_dM, double* grad);
 
 #ifdef __cplusplus

@@ -1,0 +1,104 @@
+"""GPU parity at the full benchmark workloads against the FP64 oracle.
+
+The fixtures (tests/golden/cfg_<name>.npz, written by
+tests/golden/make_configs.py with this repository's FP64 oracle) hold seeded
+entries and the diagonal of Z, the Mueller matrix and surface-current
+columns of each workload:
+  * c2: icosphere 3 refinements, 2N = 3840, ka = 5, m = 1.3116 + 0.01i,
+        lab-frame orientation, 2 polarisations;
+  * c3: hexagonal prism H/D = 1.4286, 3 refinements, 2N = 4608, ka = 5,
+        m = 1.3116, orientations 8 x 8 x 1 (128 right-hand sides);
+  * c5: hexagonal prism H/D = 0.7, 4 refinements, 2N = 18432, ka = 10,
+        m = 1.3116, orientations 8 x 8 x 1 (128 right-hand sides).
+Each runs the whole pipeline through the C ABI: assembly, plane waves, FP32
+LU on the matrix cores with FP64 refinement to 1e-10 (or FP64 LU without
+refinement), far field, orientation-averaged Mueller matrix.
+
+Tolerances:
+  * Z entry-wise: |dZ| <= 2e-6 |Z| + 1e-7 max|Z| (far pairs evaluate sin, cos
+    and exp in FP32 of an FP64-reduced phase, ~1e-7 relative per point pair);
+  * surface currents: ||x - x_ref|| / ||x_ref|| <= 1e-5 per stored column and
+    for the norms of all columns (the Z error above, amplified by the
+    conditioning of the system);
+  * Mueller: max_ij,theta |dM_ij(theta)| / M11(theta) <= 1e-4 (the gate).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+MESH = {"c2": ("sphere", 3, 1.0), "c3": ("hex", 3, 1.4286), "c5": ("hex", 4, 0.7)}
+Z_RTOL, Z_ATOL = 2e-6, 1e-7
+X_TOL = 1e-5
+MUELLER_GATE = 1e-4
+NTHETA = 181
+
+
+def _run(bem, name, precision):
+    g = golden(f"cfg_{name}.npz")
+    shape, ref, param = MESH[name]
+    v, t = bem.make_mesh(shape, ref, param)
+    # the product's mesh generator gives the oracle's mesh bit for bit
+    assert np.array_equal(t, g["tris"]) and np.array_equal(v, g["verts"])
+    ka, m = float(g["ka"]), complex(g["m"])
+    orient = tuple(int(x) for x in g["orient"])
+    print(f"{name} {precision}: 2N = {2 * len(t) * 3 // 2}, orientations {orient}", flush=True)
+    with bem.Context(0) as ctx:
+        ctx.set_mesh(v, t, 7)
+        ctx.assemble(complex(ka), ka * m, 1.0, 1.0 / abs(m))
+        n = 2 * ctx.N
+        Ze = ctx.get_Z_entries(g["Zr"], g["Zc"])
+        Zd = ctx.get_Z_entries(np.arange(n), np.arange(n))
+        if orient == (0, 0, 0):
+            RT, w = bem.lab_single_orientation()
+        else:
+            RT, w = bem.orientations(*orient)
+        kh, e0 = bem.planewave_orient_rhs(RT)
+        ctx.rhs_planewave(ka, 1.0, kh, e0)
+        assert ctx.factor(precision) == 0
+        if precision == "fp32":
+            X, st = ctx.solve(None, ir_tol=1e-10, ir_max=10, nrhs=len(kh))
+        else:
+            X, st = ctx.solve(None, ir_max=0, nrhs=len(kh))
+        M = ctx.orient_mueller(ka, 1.0, RT, w, np.linspace(0.0, np.pi, NTHETA))
+    return g, Ze, Zd, X, st, M
+
+
+def _z_errors(g, Ze, Zd):
+    zmax = float(g["Zmax"])
+    bad = np.abs(Ze - g["Zv"]) > Z_RTOL * np.abs(g["Zv"]) + Z_ATOL * zmax
+    badd = np.abs(Zd - g["Zdiag"]) > Z_RTOL * np.abs(g["Zdiag"]) + Z_ATOL * zmax
+    err = max(np.abs(Ze - g["Zv"]).max(), np.abs(Zd - g["Zdiag"]).max()) / zmax
+    return int(bad.sum() + badd.sum()), err
+
+
+def _x_errors(g, X):
+    cols = g["Xcols"]
+    col_err = max(np.linalg.norm(X[:, c] - g["X"][:, i]) / np.linalg.norm(g["X"][:, i]) for i, c in enumerate(cols))
+    norm_err = (np.abs(np.linalg.norm(X, axis=0) - g["Xnorm"]) / g["Xnorm"]).max()
+    return col_err, norm_err
+
+
+def _mueller_err(M, Mr):
+    return float((np.abs(M - Mr) / Mr[0, 0][None, None, :]).max())
+
+
+@pytest.mark.parametrize("name,precision", [
+    ("c2", "fp32"), ("c2", "fp64"),
+    ("c3", "fp32"), ("c3", "fp64"),
+    ("c5", "fp32"), ("c5", "fp64"),
+])
+def test_workload_vs_oracle(bem, name, precision):
+    g, Ze, Zd, X, st, M = _run(bem, name, precision)
+    nbad, zerr = _z_errors(g, Ze, Zd)
+    col_err, norm_err = _x_errors(g, X)
+    merr = _mueller_err(M, g["M"])
+    print(f"{name} {precision}: Z max|dZ|/max|Z| {zerr:.2e} ({nbad} outside), currents {col_err:.2e} "
+          f"(norms {norm_err:.2e}), Mueller {merr:.2e}, refinement {st}", flush=True)
+    assert nbad == 0, zerr
+    assert col_err <= X_TOL and norm_err <= X_TOL, (col_err, norm_err)
+    assert merr <= MUELLER_GATE, merr
+    if precision == "fp32":
+        assert st["converged"] and st["residual"] <= 1e-10

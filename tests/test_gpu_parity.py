@@ -8,9 +8,10 @@ from conftest import golden, python_to_sorted
 
 pytestmark = pytest.mark.gpu
 
-# normwise backward error bound of an unrefined FP64 solve (about 20 eps; see
-# test_lu_fp64_vs_numpy)
-BE_FP64 = 5e-15
+# normwise backward error bound of an unrefined FP64 solve, max|ZX - B| /
+# (max|Z| max|X| n): about 10 eps, where LAPACK reaches 0.5-3.3e-16 on the
+# seeded systems of these tests (see test_lu_fp64_vs_numpy)
+BE_FP64 = 2e-15
 
 
 @pytest.fixture(scope="module")
@@ -98,12 +99,12 @@ def test_lu_fp64_vs_numpy(ctx, n, nrhs):
     Xr = np.linalg.solve(Z, B)
     cond = np.linalg.cond(Z)
     assert np.abs(X - Xr).max() <= 1e-13 * cond * np.abs(Xr).max()
-    # normwise backward error; a stable pivoted LU gives O(n eps).  The solves
-    # apply explicit inverses of the 256 x 256 diagonal blocks (GEMMs instead
-    # of substitutions), which adds a factor of their conditioning: about
-    # 10 eps here, where LAPACK's substitutions reach about 1 eps
+    # normwise backward error; a stable pivoted LU gives O(n eps).  The FP64
+    # factorisation and solves substitute with the diagonal blocks (no
+    # explicit inverse), as Zgetrf / Zgetrs do
     be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
-    assert be < BE_FP64
+    print(f"n={n}: FP64 backward error {be:.2e}")
+    assert be < BE_FP64, be
     # one FP64 refinement sweep brings it to LAPACK's level
     X1, st1 = ctx.solve(B, ir_tol=0.0, ir_max=1, allow_noconv=True)
     be1 = np.abs(Z @ X1 - B).max() / (np.abs(Z).max() * np.abs(X1).max() * n)
@@ -330,9 +331,10 @@ def test_factor_trials_pick_a_tile_and_keep_results(ctx):
         assert ms[info["large"]] <= 1.03 * min(ms.values()) + 1e-9, info
         X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
         be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
-        # partial pivoting's growth on a random matrix rises with n: at
-        # n = 8192 the unrefined FP64 solve sits near 2e-15
-        assert be < 1e-14, (prec, be)
+        # partial pivoting's growth on a random matrix rises with n: LAPACK
+        # gives 7.6e-16 on this system
+        print(f"n={n} {prec}: backward error {be:.2e}")
+        assert be < (3e-15 if prec == "fp64" else 1e-14), (prec, be)
         # a second factorisation reuses the choice (no further trials)
         assert ctx.factor(prec) == 0
         assert ctx.gemm_plan_info(prec) == info
