@@ -376,6 +376,7 @@ class Context:
         return B.T.copy() if B is not None else None
 
     def farfield(self, cJ, cM, r_hats, k_ext: complex, eta_ext: float, n_calls: int, n_orient: int):
+        """Far field of n_calls coefficient sets (bem_farfield)."""
         cJ = np.ascontiguousarray(cJ, dtype=np.complex128)
         cM = np.ascontiguousarray(cM, dtype=np.complex128)
         rh = _cd(r_hats).reshape(n_orient, -1, 3)
@@ -433,16 +434,22 @@ class Context:
         p = {"fp32": 0, "fp64": 1}[prec]
         self._check(self._L.bem_gemm_plan(self._h, p, int(large), int(skinny), int(cap)), "gemm_plan")
 
+    # configurations of the FP32 trailing-update kernel (k_cupdate_nt): block
+    # tile, "m2": two blocks per CU instead of three, "k32": K slice of 32
+    UPDATE_TILES32 = ("64x64", "64x64m2", "128x64", "64x64k32")
+
     def gemm_plan_info(self, prec):
-        """The current GEMM plan of one precision: tile names, grid cap, and
-        the trial factorisation time per configuration (ms; 0 = not run)."""
+        """The current GEMM plan of one precision: tile names, grid cap, the
+        trailing update's configuration ("update") and the trial factorisation
+        time per trailing-update configuration (ms; 0 = not run)."""
         tiles = self.GEMM_TILES if prec == "fp32" else self.GEMM_TILES64
-        out = np.zeros(6 + len(tiles))
+        upd = self.UPDATE_TILES32 if prec == "fp32" else self.GEMM_TILES64
+        out = np.zeros(12)
         self._check(self._L.bem_gemm_plan_get(self._h, {"fp32": 0, "fp64": 1}[prec], _p(out), len(out)),
                     "gemm_plan_get")
         return {"large": tiles[int(out[0])], "skinny": tiles[int(out[1])], "panel": tiles[int(out[2])],
-                "cap": int(out[3]), "tuned": bool(out[4]), "trialed": bool(out[5]),
-                "trial_factor_ms": {t: float(v) for t, v in zip(tiles, out[6:]) if v > 0}}
+                "cap": int(out[3]), "tuned": bool(out[4]), "trialed": bool(out[5]), "update": upd[int(out[11])],
+                "trial_factor_ms": {t: float(v) for t, v in zip(upd, out[6:11]) if v > 0}}
 
     def device_copy(self, nbytes):
         """One plain device-to-device copy kernel of nbytes (a known transfer

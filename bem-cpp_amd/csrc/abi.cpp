@@ -313,6 +313,7 @@ int bem_gemm_plan_get(bem_ctx* c, int prec, double* out, int nout) {
     v[k++] = p.tuned ? 1.0 : 0.0;
     v[k++] = p.trialed ? 1.0 : 0.0;
     for (double t : p.trial_ms) v[k++] = t;
+    v[k++] = prec == 0 ? p.nt : p.large;  // the trailing update's configuration
     const int m = std::min(k, nout);
     for (int i = 0; i < m; i++) out[i] = v[i];
     return m;

@@ -98,6 +98,7 @@ struct GemmPlan {
     static constexpr int kMaxCfgs = 5;
     int large = 0, skinny = 2;
     int panel = 3;                           // products of the panel stream (smallest LDS footprint)
+    int nt = 0;                              // FP32 trailing update: configuration of k_cupdate_nt
     long fill = 512;
     bool tuned = false;
     int npad = 0;                            // system size the plan was tuned for
@@ -107,8 +108,9 @@ struct GemmPlan {
     long cap = 0;                            // grid of capped launches; 0: from occ and cus
     int sk_cfg = -1;                         // split-K products: configuration timed fastest for
     int sk_shape[4] = {0, 0, 0, 0};          //   this (M, N, K, role)
-    bool trialed = false;                    // `large` chosen by whole trial factorisations
-    double trial_ms[kMaxCfgs] = {};          // their times per configuration (0: not run)
+    bool trialed = false;                    // trailing-update tile chosen by whole trial factorisations
+    double trial_ms[kMaxCfgs] = {};          // their times per configuration (0: not run): FP32 over `nt`,
+                                             //   FP64 over `large`
     double alone_tflops[kMaxCfgs] = {};      // trailing-update shape timed alone (autotuner)
     GemmPlan() = default;
     GemmPlan(int l, int s) : large(l), skinny(s) {}

@@ -304,11 +304,11 @@ __global__ __launch_bounds__(256) void k_mueller_avg(int n_orient, int ntheta, c
 // Far field of n_calls coefficient sets at the given directions into dFv.
 int farfield_core(bem_ctx* c, const double* Jr, const double* Ji, const double* Mr, const double* Mi, long sc, long sn,
                   const double* drh, double kr, double ki, double eta, int n_calls, int n_orient, int ndir,
-                  double* dFv, char* scratch) {
+                  double* dFv, char* work) {
     const DevMesh& d = c->dm;
     const long P = (long)d.T * d.Nq;
     const int nchunk = (int)((P + kChunk - 1) / kChunk);
-    double* cur = reinterpret_cast<double*>(scratch);
+    double* cur = reinterpret_cast<double*>(work);
     double* part = cur + P * n_calls * 12;
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_current, dim3((unsigned)((P * n_calls + 255) / 256)), dim3(256), 0, s, d.T, d.Nq, n_calls,

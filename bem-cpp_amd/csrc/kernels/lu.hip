@@ -29,6 +29,7 @@
 
 #include "../bem_internal.h"
 #include "gemm.h"
+#include "gemm_nt.h"
 #include "trsm.h"
 
 namespace {
@@ -526,6 +527,32 @@ __global__ __launch_bounds__(256) void k_copy_rows(int rows, int cols, const T* 
     Di[(long)r * ldd + c] = Si[(long)r * lds + c];
 }
 
+// U12 (NB x nc in the work buffer, ld lds) into the matrix rows (ld ldd) and
+// its transpose Ut (nc x NB, ld NB) for the FP32 trailing update, whose
+// operands are both k-contiguous (gemm_nt.h).  32 x 32 tiles through LDS;
+// grid (nc / 32, NB / 32).
+__global__ __launch_bounds__(256) void k_u12_out(const float* __restrict__ Sr, const float* __restrict__ Si, long lds,
+                                                 float* __restrict__ Dr, float* __restrict__ Di, long ldd,
+                                                 float* __restrict__ Utr, float* __restrict__ Uti) {
+    __shared__ float tr[32][33], ti[32][33];
+    const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int r = ty; r < 32; r += 8) {
+        const float vr = Sr[(long)(k0 + r) * lds + c0 + tx], vi = Si[(long)(k0 + r) * lds + c0 + tx];
+        Dr[(long)(k0 + r) * ldd + c0 + tx] = vr;
+        Di[(long)(k0 + r) * ldd + c0 + tx] = vi;
+        tr[r][tx] = vr;
+        ti[r][tx] = vi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = ty; r < 32; r += 8) {
+        Utr[(long)(c0 + r) * NB + k0 + tx] = tr[tx][r];
+        Uti[(long)(c0 + r) * NB + k0 + tx] = ti[tx][r];
+    }
+}
+
 // Work buffers of one factorisation (device).
 template <typename T>
 struct LuWork {
@@ -536,6 +563,7 @@ struct LuWork {
     int* pl;         // row-pair lists of the 8 steps of an outer panel (PLS ints each), two sets
     T *d11r, *d11i;  // factored 32 x 32 diagonal block
     T *ubr, *ubi;    // 256 x n U12 buffer
+    T *utr, *uti;    // n x 256 transpose of U12 (FP32 trailing update)
     T *tr, *ti;      // 128 x 128 product of the L11^{-1} doubling
 };
 
@@ -655,6 +683,23 @@ void update_columns(bem_ctx* c, hipStream_t us, T* Ar, T* Ai, const T* Lir, cons
         kt_close(c, kt, 4.0 * (double)NB * NB * nc, 2.0 * sizeof(T) * ((double)NB * NB + 2.0 * NB * nc));
     } else {
         cgemm_ctx<T>(c, NB, nc, NB, Lir + io, Lii + io, NB, Ar + o12, Ai + o12, ld, ubr, ubi, ld, 1, us);
+        // FP32: the trailing update on the k-contiguous kernel (U12 transposed
+        // beside its copy into the matrix), when the shape fits it
+        const GemmPlan& plan = c->gemm_plan[0];
+        float* utr = w.utr + (long)(c0 - (k0 + NB)) * NB;
+        float* uti = w.uti + (long)(c0 - (k0 + NB)) * NB;
+        if (cupdate_nt_fits(plan.nt, nr, nc, NB, Ar + o21, Ai + o21, ld, utr, uti, NB, Ar + o22, Ai + o22, ld)) {
+            int kt = kt_mark(c, KS_OTHER, us);
+            hipLaunchKernelGGL(k_u12_out, dim3((unsigned)(nc / 32), NB / 32), dim3(256), 0, us, ubr, ubi, ld,
+                               Ar + o12, Ai + o12, ld, utr, uti);
+            kt_close(c, kt, 0.0, 6.0 * sizeof(T) * (double)NB * nc);
+            kt = kt_mark(c, KS_UPDATE32, us);
+            cupdate_nt(plan.nt, us, nr, nc, NB, Ar + o21, Ai + o21, ld, utr, uti, NB, Ar + o22, Ai + o22, ld,
+                       headroom ? gemm_headroom_cap(plan, plan.large) : 0);
+            kt_close(c, kt, 6.0 * (double)nr * nc * NB,
+                     2.0 * sizeof(T) * ((double)nr * NB + (double)NB * nc + 2.0 * nr * nc));
+            return;
+        }
     }
     cgemm_ctx<T>(c, nr, nc, NB, Ar + o21, Ai + o21, ld, ubr, ubi, ld, Ar + o22, Ai + o22, ld, 0, us,
                  GEMM_UPDATE | (headroom ? GEMM_HEADROOM : 0));
@@ -756,7 +801,7 @@ int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     const size_t ncnt = 2 * (size_t)nsets0 + 64;
     const size_t nint = 64 + ncnt + 2 * (NB / PB) * PLS + 2 * (size_t)nsets0 * PB;
     const size_t bytes = nint * sizeof(int) + 256 + 4 * (size_t)nsets0 * PB * PB * sizeof(T) +
-                         2 * (size_t)PB * PB * sizeof(T) + 2 * (size_t)NB * n * sizeof(T) +
+                         2 * (size_t)PB * PB * sizeof(T) + 4 * (size_t)NB * n * sizeof(T) +
                          2 * (size_t)(NB / 2) * (NB / 2) * sizeof(T) + 256;
     void* scr = nullptr;
     if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
@@ -773,7 +818,9 @@ int factor_prec(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes) {
     w.d11i = w.d11r + PB * PB;
     w.ubr = w.d11i + PB * PB;
     w.ubi = w.ubr + (size_t)NB * n;
-    w.tr = w.ubi + (size_t)NB * n;
+    w.utr = w.ubi + (size_t)NB * n;
+    w.uti = w.utr + (size_t)NB * n;
+    w.tr = w.uti + (size_t)NB * n;
     w.ti = w.tr + (size_t)(NB / 2) * (NB / 2);
     BEM_HIP(c, hipMemsetAsync(w.info, 0, (64 + ncnt) * sizeof(int), c->stream));
     // the doubling writes only the diagonal and lower blocks of each L11^{-1}
@@ -837,8 +884,12 @@ int factor_trials(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes, bool*
     }
     *ran = true;
     int rc = 0;
-    for (int cfg = 0; cfg < kGemmTrialCfgs && !rc; cfg++) {
-        plan.large = cfg;
+    // FP32 tries the configurations of its trailing-update kernel (k_cupdate_nt),
+    // FP64 the large tile of k_cgemm
+    constexpr bool f32 = sizeof(T) == 4;
+    const int ncfg = f32 ? kNtCfgs : kGemmTrialCfgs;
+    for (int cfg = 0; cfg < ncfg && !rc; cfg++) {
+        (f32 ? plan.nt : plan.large) = cfg;
         if ((rc = load_working_copy<T>(c, Ar, Ai))) break;
         (void)hipEventRecord(e0, c->stream);
         if ((rc = factor_prec<T>(c, Ar, Ai, inv, inv_bytes))) break;
@@ -854,15 +905,18 @@ int factor_trials(bem_ctx* c, T* Ar, T* Ai, void*& inv, size_t& inv_bytes, bool*
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // the fastest trial; among trials within 3 % of it (run-to-run noise),
-    // the configuration with the higher rate timed alone
+    // FP64 takes the configuration with the higher rate timed alone, FP32 the
+    // first (configuration 0 is the fastest alone)
     int best = 0;
-    for (int cfg = 1; cfg < kGemmTrialCfgs; cfg++)
+    for (int cfg = 1; cfg < ncfg; cfg++)
         if (plan.trial_ms[cfg] < plan.trial_ms[best]) best = cfg;
     const double lim = 1.03 * plan.trial_ms[best];
     int pick = best;
-    for (int cfg = 0; cfg < kGemmTrialCfgs; cfg++)
-        if (plan.trial_ms[cfg] <= lim && plan.alone_tflops[cfg] > plan.alone_tflops[pick]) pick = cfg;
-    plan.large = pick;
+    for (int cfg = 0; cfg < ncfg; cfg++) {
+        if (plan.trial_ms[cfg] > lim) continue;
+        if (f32 ? cfg < pick : plan.alone_tflops[cfg] > plan.alone_tflops[pick]) pick = cfg;
+    }
+    (f32 ? plan.nt : plan.large) = pick;
     if (rc) return rc;
     plan.trialed = true;
     gemm_cache_store(c, sizeof(T) == 4);

@@ -183,116 +183,138 @@ __global__ __launch_bounds__(64) void k_inv_lower32(const T* __restrict__ Fr, co
     }
 }
 
+// Value of lane l (uniform) of the wave, broadcast through the scalar unit.
+__device__ __forceinline__ double wave_bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float wave_bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 // Solves T X = B by substitution for one 256 x 256 triangular block T (unit
 // lower: forward; non-unit upper: backward) and the columns of B, CW columns
-// per block (blockIdx.x; ncols a multiple of CW).  This is the backward-
-// stable form of the FP64 solves and of U12 = L11^{-1} A12 in the FP64
-// factorisation (no explicit inverse is formed).
-//   Thread t keeps row t of X in registers.  The triangle is staged in
-// 32-column slabs.  Inside a slab the 32 rows are solved by the one wave that
-// holds them, with the finished row of each step passed through LDS inside
-// the wave (LDS operations of one wave complete in issue order, so the step
-// needs no workgroup barrier); then a barrier publishes the slab's 32 rows and
-// every row outside the slab subtracts their contribution.  Slabs are 16
-// columns wide in FP64 and 32 in FP32 (about 68 KB of LDS either way, so the
-// kernel fits on a CU beside a running trailing update).  Three barriers per
-// slab instead of one or two per row.  In the upper case the
-// pivot row is divided by CW lanes at once, one entry each.  B and X may be
-// the same array (each block reads its columns before it writes them).
-template <typename T, int CW, bool LOWER>
+// per block (blockIdx.x; the column count a multiple of CW).  This is the
+// backward-stable form of the FP64 solves and of U12 = L11^{-1} A12 in the
+// FP64 factorisation (no explicit inverse is formed).
+//   Thread t keeps row t of X (CW columns) and, slab by slab, its own row of
+// T (S columns) in registers; the next slab's row is loaded while the
+// current one is used, so the triangle never passes through LDS.  Inside a
+// slab the S rows are solved by the one wave that holds them: the finished
+// row of each step goes to every lane of the wave through the scalar unit
+// (readlane), with no LDS round trip and no barrier.  Then the slab's rows go
+// to LDS, one barrier publishes them (the LDS rows are double buffered by slab
+// parity, so that is the only barrier of a slab) and every row outside the
+// slab subtracts their contribution.  The upper case scales the pivot row by
+// the reciprocal of its diagonal entry, formed by every lane of the slab for
+// its own row before the steps (off the critical path of the steps).  B and X
+// may be the same array (each block reads its columns before it writes them).
+template <typename T, int CW, int S, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsm_rhs(const T* __restrict__ Tr, const T* __restrict__ Ti, long ldt,
                                                   const T* Br, const T* Bi, long ldb, T* Xr, T* Xi, long ldx) {
     __builtin_amdgcn_s_setprio(3);  // U12 of the next panel is on the factorisation's critical path
-    constexpr int R = 256, S = sizeof(T) == 8 ? 16 : 32;
-    __shared__ T sr[R][S + 1], si[R][S + 1];
-    __shared__ T pr[S][CW], pi[S][CW];
+    constexpr int R = 256, NS = R / S;
+    static_assert(S <= 64 && 64 % S == 0, "a slab lies inside one wave");
+    __shared__ T pr[2][S][CW], pi[2][S][CW];
     const int t = threadIdx.x, c0 = blockIdx.x * CW;
+    const T* rowr = Tr + (long)t * ldt;
+    const T* rowi = Ti + (long)t * ldt;
     T xr[CW], xi[CW];
 #pragma unroll
     for (int c = 0; c < CW; c++) {
         xr[c] = Br[(long)t * ldb + c0 + c];
         xi[c] = Bi[(long)t * ldb + c0 + c];
     }
+    auto slab0 = [](int sI) { return LOWER ? sI * S : R - S * (sI + 1); };
+    T ar[S], ai[S], nr[S], ni[S];
+#pragma unroll
+    for (int q = 0; q < S; q++) {
+        ar[q] = rowr[slab0(0) + q];
+        ai[q] = rowi[slab0(0) + q];
+    }
 #pragma unroll 1
-    for (int sI = 0; sI < R / S; sI++) {
-        const int s = LOWER ? sI * S : R - S * (sI + 1);
-        // rows of this slab's columns that are read: below the slab's top row
-        // (lower) or above its bottom row (upper)
-        const int r0 = LOWER ? s : 0, r1 = LOWER ? R : s + S;
-        __syncthreads();  // the previous slab's readers are done with sr / pr
-        for (int e = r0 * S + t; e < r1 * S; e += 256) {
-            const int r = e / S, cc = e % S;
-            sr[r][cc] = Tr[(long)r * ldt + s + cc];
-            si[r][cc] = Ti[(long)r * ldt + s + cc];
+    for (int sI = 0; sI < NS; sI++) {
+        const int s = slab0(sI), buf = sI & 1;
+        if (sI + 1 < NS) {
+            const int sn = slab0(sI + 1);
+#pragma unroll
+            for (int q = 0; q < S; q++) {
+                nr[q] = rowr[sn + q];
+                ni[q] = rowi[sn + q];
+            }
         }
-        __syncthreads();
         if ((t >> 6) == (s >> 6)) {  // the wave that holds rows s .. s + S - 1
-            const int lane0 = t & ~63;
-#pragma unroll 1
+            T rr = T(1), ri = T(0);   // upper: 1 / T[t][t] for the rows of the slab
+            if (!LOWER) {
+                T dr = T(1), di = T(0);
+#pragma unroll
+                for (int q = 0; q < S; q++)
+                    if (t == s + q) {
+                        dr = ar[q];
+                        di = ai[q];
+                    }
+                const T inv = T(1) / (dr * dr + di * di);
+                rr = dr * inv;
+                ri = -di * inv;
+            }
+#pragma unroll
             for (int k = 0; k < S; k++) {
-                const int l = LOWER ? s + k : s + S - 1 - k, q = l - s;
-                if (t == l) {
+                const int q = LOWER ? k : S - 1 - k, l = s + q;
+                if (!LOWER && t == l) {
 #pragma unroll
                     for (int c = 0; c < CW; c++) {
-                        pr[q][c] = xr[c];
-                        pi[q][c] = xi[c];
+                        const T a = xr[c], b = xi[c];
+                        xr[c] = a * rr - b * ri;
+                        xi[c] = a * ri + b * rr;
                     }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (!LOWER) {
-                    if (t - lane0 < CW) {
-                        const int c = t - lane0;
-                        T qr, qi;
-                        cdiv(pr[q][c], pi[q][c], sr[l][q], si[l][q], qr, qi);
-                        pr[q][c] = qr;
-                        pi[q][c] = qi;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    if (t == l) {
+                T ur[CW], ui[CW];
 #pragma unroll
-                        for (int c = 0; c < CW; c++) {
-                            xr[c] = pr[q][c];
-                            xi[c] = pi[q][c];
-                        }
-                    }
+                for (int c = 0; c < CW; c++) {
+                    ur[c] = wave_bcast(xr[c], l & 63);
+                    ui[c] = wave_bcast(xi[c], l & 63);
                 }
                 if (LOWER ? (t > l && t < s + S) : (t >= s && t < l)) {
-                    const T a = sr[t][q], b = si[t][q];
+                    const T a = ar[q], b = ai[q];
                     const T na = -a, nb = -b;
 #pragma unroll
                     for (int c = 0; c < CW; c++) {
-                        const T ur = pr[q][c], ui = pi[q][c];
-                        xr[c] = fmadd(na, ur, xr[c]);
-                        xr[c] = fmadd(b, ui, xr[c]);
-                        xi[c] = fmadd(na, ui, xi[c]);
-                        xi[c] = fmadd(nb, ur, xi[c]);
+                        xr[c] = fmadd(na, ur[c], xr[c]);
+                        xr[c] = fmadd(b, ui[c], xr[c]);
+                        xi[c] = fmadd(na, ui[c], xi[c]);
+                        xi[c] = fmadd(nb, ur[c], xi[c]);
                     }
                 }
-                // the next step's row goes to another LDS row, but the
-                // division of the upper case rewrites this one: keep the
-                // reads of this step ahead of the next step's writes
-                __builtin_amdgcn_wave_barrier();
+            }
+            if (t >= s && t < s + S) {
+#pragma unroll
+                for (int c = 0; c < CW; c++) {
+                    pr[buf][t - s][c] = xr[c];
+                    pi[buf][t - s][c] = xi[c];
+                }
             }
         }
-        __syncthreads();  // pr holds the slab's S solved rows
+        __syncthreads();  // pr[buf] holds the slab's S solved rows
         if (LOWER ? (t >= s + S) : (t < s)) {
-#pragma unroll 4
+#pragma unroll
             for (int q = 0; q < S; q++) {
-                const T a = sr[t][q], b = si[t][q];
+                const T a = ar[q], b = ai[q];
                 const T na = -a, nb = -b;
 #pragma unroll
                 for (int c = 0; c < CW; c++) {
-                    const T ur = pr[q][c], ui = pi[q][c];
+                    const T ur = pr[buf][q][c], ui = pi[buf][q][c];
                     xr[c] = fmadd(na, ur, xr[c]);
                     xr[c] = fmadd(b, ui, xr[c]);
                     xi[c] = fmadd(na, ui, xi[c]);
                     xi[c] = fmadd(nb, ur, xi[c]);
                 }
             }
+        }
+#pragma unroll
+        for (int q = 0; q < S; q++) {
+            ar[q] = nr[q];
+            ai[q] = ni[q];
         }
     }
 #pragma unroll
@@ -302,21 +324,33 @@ __global__ __launch_bounds__(256) void k_trsm_rhs(const T* __restrict__ Tr, cons
     }
 }
 
-// Column-chunk width of k_trsm_rhs (FP64: 16 registers of X per thread).
-constexpr int kTrsmRhsCW = 8;
-
 // T X = B for the 256 x 256 triangle at (Tr, Ti) and ncols columns (a
-// multiple of kTrsmRhsCW) on stream s.
+// multiple of 8) on stream s.  Few columns (the solves' right-hand sides, the
+// 256 columns of the next panel): 2 columns and 16-row slabs per block, so the
+// latency-bound chain of steps is as short as it gets; many columns (the rest
+// of a panel's U12): 8 columns and 8-row slabs, where throughput counts
+// (measured on MI355X, FP64: 48 / 61 us lower / upper at 128 columns, 274 /
+// 322 us at 9216).
 template <typename T>
 inline void trsm_rhs(hipStream_t s, bool lower, const T* Tr, const T* Ti, long ldt, const T* Br, const T* Bi,
                      long ldb, T* Xr, T* Xi, long ldx, int ncols) {
-    const dim3 grid((unsigned)(ncols / kTrsmRhsCW));
-    if (lower)
-        hipLaunchKernelGGL((k_trsm_rhs<T, kTrsmRhsCW, true>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr, Xi,
-                           ldx);
-    else
-        hipLaunchKernelGGL((k_trsm_rhs<T, kTrsmRhsCW, false>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr,
-                           Xi, ldx);
+    if (ncols <= 512) {
+        const dim3 grid((unsigned)(ncols / 2));
+        if (lower)
+            hipLaunchKernelGGL((k_trsm_rhs<T, 2, 16, true>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr, Xi,
+                               ldx);
+        else
+            hipLaunchKernelGGL((k_trsm_rhs<T, 2, 16, false>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr, Xi,
+                               ldx);
+    } else {
+        const dim3 grid((unsigned)(ncols / 8));
+        if (lower)
+            hipLaunchKernelGGL((k_trsm_rhs<T, 8, 8, true>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr, Xi,
+                               ldx);
+        else
+            hipLaunchKernelGGL((k_trsm_rhs<T, 8, 8, false>), grid, dim3(256), 0, s, Tr, Ti, ldt, Br, Bi, ldb, Xr, Xi,
+                               ldx);
+    }
 }
 
 // Column-chunk width per precision (registers per thread: 2 CW values).

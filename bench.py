@@ -103,39 +103,94 @@ def mueller_err(M, Mref):
     return float((np.abs(M - Mref) / np.abs(Mref[0, 0])[None, None, :]).max())
 
 
-def cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N_full):
-    """FP64 oracle on the host cores, on a bounded sample, scaled to s/orient of
-    the full workload: assembly (point pairs ~ N^2) from the mesh two
-    refinements coarser, factor + 128-RHS solve (~ n^3) from a random system of
-    n = 3072, far field (~ points per orientation) from two orientations of the
-    coarser mesh."""
+def lapack_lu_solve(n, nrhs):
+    """Seconds of scipy.linalg.lu_factor + lu_solve (LAPACK zgetrf / zgetrs,
+    the calls of the reference's CPU twin, python/bem_core.py:1379-1388) on a
+    random complex n x n system with nrhs right-hand sides, and the BLAS
+    library with its thread count."""
+    import scipy.linalg
+    lib = "LAPACK zgetrf/zgetrs (scipy.linalg.lu_factor/lu_solve)"
+    try:
+        from threadpoolctl import threadpool_info
+        info = [d for d in threadpool_info() if d.get("user_api") == "blas"]
+        if info:
+            lib += f" on {info[0].get('internal_api')} {info[0].get('version')}, {info[0].get('num_threads')} threads"
+    except Exception:
+        pass
+    rs = np.random.RandomState(42)
+    W = rs.randn(512, 512) + 1j * rs.randn(512, 512)
+    scipy.linalg.lu_factor(W, check_finite=False)  # starts the BLAS threads
+    Z = rs.randn(n, n) + 1j * rs.randn(n, n)
+    B = rs.randn(n, nrhs) + 1j * rs.randn(n, nrhs)
+    t0 = time.perf_counter()
+    lu = scipy.linalg.lu_factor(Z, overwrite_a=True, check_finite=False)
+    scipy.linalg.lu_solve(lu, B, check_finite=False)
+    return time.perf_counter() - t0, lib
+
+
+def cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N_full, full=False):
+    """The reference's CPU path on the host cores, s/orient of the workload:
+    assembly, singular terms, RHS, far field and Mueller by the FP64 oracle
+    (OpenMP); the factorisation and the multi-RHS solve by LAPACK, as the
+    reference's CPU twin does.  Bounded sample (default): assembly (point
+    pairs ~ N^2) on the mesh two refinements coarser, factor + solve (~ n^3)
+    of a random system of n = 6144, far field (~ points per orientation) of
+    two orientations of the coarser mesh, each scaled to the full size.
+    full=True times the whole workload instead (small workloads only)."""
     from oracle import oracle
 
     threads = oracle.num_threads()
-    ref_s = max(ref - 2, 0)
+    n_full = 2 * N_full
+    # LAPACK first: the oracle's OpenMP threads keep spinning for a while
+    # after each parallel region and would compete with the BLAS threads
+    n_s = n_full if full else min(6144, n_full)
+    t_lu, lapack = lapack_lu_solve(n_s, 2 * n_orient)
+    t_lu *= (n_full / n_s) ** 3
+    ref_s = ref if full else max(ref - 2, 0)
     p = oracle.Problem.from_shape(shape, ref_s, param)
     scale_pts = (N_full / p.N)
     t0 = time.perf_counter()
     p.assemble_pmchwt(complex(ka), ka * m, 1.0, 1.0 / abs(m))
     t_asm = (time.perf_counter() - t0) * scale_pts ** 2
-    n_s = 3072
-    rs = np.random.RandomState(42)
-    Z = rs.randn(n_s, n_s) + 1j * rs.randn(n_s, n_s)
-    B = rs.randn(n_s, 2 * n_orient) + 1j * rs.randn(n_s, 2 * n_orient)
-    t0 = time.perf_counter()
-    oracle.lu_solve(Z, B)
-    t_lu = (time.perf_counter() - t0) * (2.0 * N_full / n_s) ** 3
-    info, Mo, X = p.dense_orient(ka, m, 2, 1, 1, ntheta=ntheta)
-    t_ff_per_orient = p.last_timings()["farfield_s"] / 2 * scale_pts
+    n_ff = n_orient if full else 2
+    info, Mo, X = p.dense_orient(ka, m, n_ff, 1, 1, ntheta=ntheta)
+    t_ff_per_orient = p.last_timings()["farfield_s"] / n_ff * scale_pts
     total = t_asm + t_lu + t_ff_per_orient * n_orient
-    return {
-        "value": total / n_orient, "unit": "s/orient", "cores": threads, "kind": "port",
-        "sample": (f"FP64 oracle (oracle/bem_oracle.cpp, OpenMP {threads} threads): assembly of {shape} ref {ref_s} "
-                   f"(N={p.N}) x (N/{p.N})^2, LU + {2 * n_orient}-RHS solve of a random n={n_s} system x "
-                   f"(2N/{n_s})^3, far field of 2 orientations on ref {ref_s} x points ratio; extrapolated to "
-                   f"N={N_full}: assembly {t_asm:.3g} s, LU+solve {t_lu:.3g} s, far field "
-                   f"{t_ff_per_orient * n_orient:.3g} s"),
-    }
+    if full:
+        sample = (f"full workload: FP64 oracle (oracle/bem_oracle.cpp, OpenMP {threads} threads) assembly "
+                  f"{t_asm:.3g} s and far field of {n_orient} orientations {t_ff_per_orient * n_orient:.3g} s; "
+                  f"{lapack} factor + {2 * n_orient}-RHS solve at n={n_s} {t_lu:.3g} s")
+    else:
+        sample = (f"FP64 oracle (oracle/bem_oracle.cpp, OpenMP {threads} threads): assembly of {shape} ref {ref_s} "
+                  f"(N={p.N}) x (N/{p.N})^2, far field of 2 orientations on ref {ref_s} x points ratio; "
+                  f"{lapack} factor + {2 * n_orient}-RHS solve of a random n={n_s} system x (2N/{n_s})^3; "
+                  f"extrapolated to N={N_full}: assembly {t_asm:.3g} s, LU+solve {t_lu:.3g} s, far field "
+                  f"{t_ff_per_orient * n_orient:.3g} s")
+    return {"value": total / n_orient, "unit": "s/orient", "cores": threads, "kind": "port", "sample": sample}
+
+
+def oracle_fixture_errors(ctx, workload, M32, X32):
+    """Errors of this run against the FP64 oracle's fixture of the workload
+    (tests/golden/cfg_<workload>.npz, data written by the oracle): Z entry-wise
+    on 2000 seeded entries and the diagonal (max |dZ| / max|Z|), surface
+    currents (||x - x_ref|| / ||x_ref|| over the stored columns), Mueller
+    (max_ij,theta |dM_ij| / M11).  None when there is no fixture."""
+    path = os.path.join(ROOT, "tests", "golden", f"cfg_{workload}.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    n = len(g["Zdiag"])
+    Ze = ctx.get_Z_entries(g["Zr"], g["Zc"])
+    Zd = ctx.get_Z_entries(np.arange(n), np.arange(n))
+    zmax = float(g["Zmax"])
+    z_err = max(np.abs(Ze - g["Zv"]).max(), np.abs(Zd - g["Zdiag"]).max()) / zmax
+    cols = g["Xcols"]
+    x_err = max(float(np.linalg.norm(X32[:, c] - g["X"][:, i]) / np.linalg.norm(g["X"][:, i]))
+                for i, c in enumerate(cols))
+    return {"z_err_vs_oracle": float(z_err), "current_err_vs_oracle": x_err,
+            "mueller_err_vs_oracle": mueller_err(M32, g["M"]),
+            "fixture": f"tests/golden/cfg_{workload}.npz (FP64 oracle: 2000 seeded Z entries + diagonal, "
+                       f"currents of RHS columns {list(int(c) for c in cols)}, Mueller 4x4x{g['M'].shape[2]})"}
 
 
 def main():
@@ -147,6 +202,8 @@ def main():
     ap.add_argument("--ir-tol", type=float, default=1e-10)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the CPU baseline on the whole workload instead of a bounded sample (small workloads)")
     ap.add_argument("--gemm-tile", default=None,
                     help="fix the FP32 GEMM tile (e.g. 64x64) instead of the trial factorisations; for counter runs, "
                          "whose serialised kernels distort the trials")
@@ -251,16 +308,24 @@ def main():
     plan32 = ctx.gemm_plan_info("fp32")
     fp64 = {}
     plan64 = None
+    # the FP32 + refinement solutions of the last step against the oracle's fixture
+    X32, _ = ctx.solve(None, ir_tol=args.ir_tol, ir_max=20, return_X=True, nrhs=nrhs, allow_noconv=True)
+    vs_oracle = oracle_fixture_errors(ctx, args.workload, M32, X32)
+    del X32
+    log(f"against the oracle fixture: {vs_oracle}")
     if not args.skip_fp64:
         ctx.factor("fp64")  # first FP64 factorisation of this size: tile trials + factorisation (untimed)
         plan64 = ctx.gemm_plan_info("fp64")
-        ctx.factor("fp64")
-        f64_factor = ctx.timings()["factor_ms"]
+        f64_runs = []
+        for _ in range(3):
+            ctx.factor("fp64")
+            f64_runs.append(ctx.timings()["factor_ms"])
+        f64_factor = float(np.mean(f64_runs))
         X, st64 = ctx.solve(None, ir_tol=0.0, ir_max=0, return_X=False, nrhs=nrhs)
         f64_solve = ctx.timings()["solve_ms"]
         M64 = ctx.orient_mueller(k_ext, 1.0, RT, w, theta)
         f64_ff = ctx.timings()["farfield_ms"]
-        fp64 = dict(factor_ms=f64_factor, solve_ms=f64_solve, farfield_ms=f64_ff,
+        fp64 = dict(factor_ms=f64_factor, factor_ms_runs=f64_runs, solve_ms=f64_solve, farfield_ms=f64_ff,
                     s_per_orient=(ph["rhs"] + f64_factor + f64_solve + f64_ff) / n_orient / 1e3,
                     mueller_err_fp32ir_vs_fp64=mueller_err(M32, M64))
         log(f"fp64: {fp64}")
@@ -275,16 +340,15 @@ def main():
         log(f"external FP64 LU: {external}")
 
     # roofline of the dominant kernel, the FP32 3M trailing-update GEMM of the
-    # LU (its own symbol k_cgemm<float, cfg, 1>), from the live HIP-event
-    # times of its launches over the timed steps
+    # LU (k_cupdate_nt), from the live HIP-event times of its launches over the
+    # timed steps
     dom = "gemm_fp32_update"
     launches, kms, kflops, kbytes = kstats.get(dom, (0, 0.0, 0.0, 0.0))
     if launches > 0 and kms > 0:
         ach = kflops / (kms * 1e-3) / 1e12
-        tile = plan32["large"] if plan32 else "?"
-        small = plan32["skinny"] if plan32 else "?"
-        roof = {"kernel": f"k_cgemm<float, cfg, 1> (3M complex GEMM, LU trailing updates: {tile} tile; "
-                          f"{small} / 32x32 for the late updates that {tile} cannot spread over all CUs)",
+        tile = plan32["update"] if plan32 else "?"
+        roof = {"kernel": f"k_cupdate_nt (3M complex GEMM C -= A Bt^T, both operands k-contiguous, LU trailing "
+                          f"updates, {tile} tile; k_cgemm<float, cfg, 1> for the last updates below 64 rows)",
                 "bound": "mfma",
                 "achieved": ach, "peak": PEAK_TFLOPS["gemm_fp32_3m"], "unit": "TFLOP/s",
                 "frac": ach / PEAK_TFLOPS["gemm_fp32_3m"], "traffic": measured_traffic(args.workload, dom),
@@ -325,9 +389,11 @@ def main():
         "s_per_orient_fp64_excl_assembly": fp64.get("s_per_orient"),
         "speedup_fp32ir_vs_fp64": (fp64["s_per_orient"] / s_orient_excl_asm) if fp64 else None,
         "mueller_err_vs_fp64": fp64.get("mueller_err_fp32ir_vs_fp64"),
+        "vs_oracle": vs_oracle,
         "ir_sweeps": st["iterations"], "ir_residual": st["residual"],
         "phase_ms": ph,
-        "fp64_phase_ms": {k: fp64[k] for k in ("factor_ms", "solve_ms", "farfield_ms")} if fp64 else None,
+        "fp64_phase_ms": {k: fp64[k] for k in ("factor_ms", "factor_ms_runs", "solve_ms", "farfield_ms")}
+        if fp64 else None,
         "kernels_instrumented_step": {
             k: {"launches": int(v[0]), "ms": v[1], "tflops": (v[2] / (v[1] * 1e-3) / 1e12) if v[1] else 0.0}
             for k, v in kstats_all.items() if v[0] > 0},
@@ -336,7 +402,7 @@ def main():
     }
     ctx.close()
     if not args.skip_cpu:
-        out["cpu_baseline"] = cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N)
+        out["cpu_baseline"] = cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N, full=args.cpu_full)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

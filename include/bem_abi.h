@@ -108,10 +108,13 @@ int bem_gemm_plan(bem_ctx* ctx, int prec, int large, int skinny, long cap);
 /* The GEMM plan of one precision (prec 0 FP32, 1 FP64) as doubles: large,
  * skinny and panel configuration, grid cap, tuned (0/1), chosen by trial
  * factorisations (0/1), then the trial factorisation time in ms of each
- * configuration (0 where none ran).  Systems of 8192 unknowns and more pick
- * the trailing-update configuration on their first factorisation per
- * precision by factoring once with each (no counterpart in the reference).
- * Returns the number of values written, or < 0. */
+ * trailing-update configuration (5 values, 0 where none ran), then the
+ * trailing update's configuration.  The FP32 trailing update runs on its own
+ * kernel with 4 configurations (64 x 64 with 3 or 2 blocks per CU, 128 x 64,
+ * 64 x 64 with a K slice of 32); FP64 on the large tile.  Systems of 8192
+ * unknowns and more pick it on their first factorisation per precision by
+ * factoring once with each (no counterpart in the reference).  Returns the
+ * number of values written, or < 0. */
 int bem_gemm_plan_get(bem_ctx* ctx, int prec, double* out, int nout);
 
 /* Copies `bytes` bytes (rounded down to 16) device to device with one plain

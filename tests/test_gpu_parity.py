@@ -324,11 +324,11 @@ def test_factor_trials_pick_a_tile_and_keep_results(ctx):
         assert ctx.factor(prec) == 0
         info = ctx.gemm_plan_info(prec)
         assert info["trialed"], info
-        tiles = ctx.GEMM_TILES if prec == "fp32" else ctx.GEMM_TILES64
-        assert sorted(info["trial_factor_ms"]) == sorted(tiles[:4]), info  # the 32 x 32 tile is not tried
-        # the fastest trial, or one within 3 % of it that is faster alone
+        tiles = ctx.UPDATE_TILES32 if prec == "fp32" else ctx.GEMM_TILES64[:4]  # the 32 x 32 tile is not tried
+        assert sorted(info["trial_factor_ms"]) == sorted(tiles), info
+        # the fastest trial, or one within 3 % of it
         ms = info["trial_factor_ms"]
-        assert ms[info["large"]] <= 1.03 * min(ms.values()) + 1e-9, info
+        assert ms[info["update"]] <= 1.03 * min(ms.values()) + 1e-9, info
         X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
         be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
         # partial pivoting's growth on a random matrix rises with n: LAPACK
