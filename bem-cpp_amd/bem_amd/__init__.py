@@ -92,6 +92,7 @@ def lib():
         "bem_precond_apply": ([vp, i, _dp, i, _dp], i),
         "bem_rhs_planewave": ([vp, d, d, d, _dp, _dp, i, _dp], i),
         "bem_farfield": ([vp, _dp, _dp, _dp, d, d, d, i, i, i, _dp], i),
+        "bem_farfield_noalloc": ([vp, _dp, _dp, _dp, d, d, d, i, i, i, _dp], i),
         "bem_orient_mueller": ([vp, d, d, d, _dp, _dp, i, _dp, i, i, _dp, _dp], i),
         "bem_mueller_grad": ([vp, _dp, _dp], i),
     }
@@ -375,16 +376,19 @@ class Context:
                                               _p(B) if B is not None else None), "rhs_planewave")
         return B.T.copy() if B is not None else None
 
-    def farfield(self, cJ, cM, r_hats, k_ext: complex, eta_ext: float, n_calls: int, n_orient: int):
-        """Far field of n_calls coefficient sets (bem_farfield)."""
+    def farfield(self, cJ, cM, r_hats, k_ext: complex, eta_ext: float, n_calls: int, n_orient: int,
+                 noalloc=False):
+        """Far field of n_calls coefficient sets (bem_farfield); noalloc=True
+        takes bem_farfield_noalloc (no device allocation, bounded size)."""
         cJ = np.ascontiguousarray(cJ, dtype=np.complex128)
         cM = np.ascontiguousarray(cM, dtype=np.complex128)
         rh = _cd(r_hats).reshape(n_orient, -1, 3)
         ndir = rh.shape[1]
         Fv = np.zeros((n_calls, ndir, 3), np.complex128)
         k_ext = complex(k_ext)
-        self._check(self._L.bem_farfield(self._h, _p(cJ), _p(cM), _p(rh), k_ext.real, k_ext.imag, eta_ext, n_calls,
-                                         n_orient, ndir, _p(Fv)), "farfield")
+        f = self._L.bem_farfield_noalloc if noalloc else self._L.bem_farfield
+        self._check(f(self._h, _p(cJ), _p(cM), _p(rh), k_ext.real, k_ext.imag, eta_ext, n_calls, n_orient, ndir,
+                      _p(Fv)), "farfield_noalloc" if noalloc else "farfield")
         return Fv
 
     def orient_mueller(self, k_ext: complex, eta_ext: float, RT, w, theta, gamma_mirror=False, per_orient=False):

@@ -230,6 +230,10 @@ bem_ctx* bem_create(int device) {
         delete c;
         return nullptr;
     }
+    // the device address of the no-allocation array; a failure is reported by
+    // bem_farfield_noalloc, the other entry points do not need it
+    (void)ff_noalloc_init(c);
+    c->err.clear();
     return c;
 }
 
@@ -955,6 +959,17 @@ int bem_farfield(bem_ctx* c, const double* cJ, const double* cM, const double* r
     c->t_ms[4] = tm.stop();
     BEM_HIP(c, hipMemcpy(Fv, dfv, (size_t)n_calls * ndir * 6 * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
+}
+
+int bem_farfield_noalloc(bem_ctx* c, const double* cJ, const double* cM, const double* rh, double kr, double ki,
+                         double eta, int n_calls, int n_orient, int ndir, double* Fv) {
+    if (!c || !cJ || !cM || !rh || !Fv || n_calls < 1 || n_orient < 1 || ndir < 1)
+        return bem_fail(c, BEM_ERR_ARG, "farfield_noalloc: bad arguments");
+    if ((n_calls + 1) / 2 > n_orient)
+        return bem_fail(c, BEM_ERR_ARG, "farfield_noalloc: n_calls needs ceil(n_calls/2) orientations");
+    if (!c->has_mesh) return bem_fail(c, BEM_ERR_STATE, "farfield_noalloc: no mesh");
+    (void)hipSetDevice(c->device);
+    return ff_noalloc_launch(c, cJ, cM, rh, kr, ki, eta, n_calls, n_orient, ndir, Fv);
 }
 
 int bem_orient_mueller(bem_ctx* c, double kr, double ki, double eta, const double* RT, const double* w, int n_orient,

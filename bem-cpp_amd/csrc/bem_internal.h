@@ -195,6 +195,7 @@ struct bem_ctx {
     size_t scratch_bytes = 0;
     void* part = nullptr;  // partial products of split-K GEMMs
     size_t part_bytes = 0;
+    void* noalloc_ptr = nullptr;  // device address of the static no-allocation array (bem_farfield_noalloc)
 
     double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -238,6 +239,13 @@ int rhs_launch(bem_ctx* c, double kr, double ki, double eta, const double* khat,
 int ff_launch(bem_ctx* c, const double* dJr, const double* dJi, const double* dMr, const double* dMi, int ldc,
               const double* drh, double kr, double ki, double eta, int n_calls, int n_orient, int ndir,
               double* dFv);
+// bem_farfield without device allocation: all staging in the library's static
+// per-GPU array of kNoAllocBytes (farfield.hip); ff_noalloc_init fetches its
+// device address into c->noalloc_ptr (once per context, device current).
+constexpr size_t kNoAllocBytes = 32u << 20;
+int ff_noalloc_init(bem_ctx* c);
+int ff_noalloc_launch(bem_ctx* c, const double* cJ, const double* cM, const double* rh, double kr, double ki,
+                      double eta, int n_calls, int n_orient, int ndir, double* Fv);
 int mueller_launch(bem_ctx* c, double kr, double ki, double eta, const double* RT, const double* w, int n_orient,
                    const double* theta, int ntheta, int mirror, double* M_avg, double* M_orient);
 int adjoint_launch(bem_ctx* c, const double* dL_dM, double* grad);

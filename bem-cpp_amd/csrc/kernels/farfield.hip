@@ -624,6 +624,60 @@ int rhs_dk_launch(bem_ctx* c, double* dBr, double* dBi) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// The far field without device allocation (bem_farfield_noalloc): every
+// device buffer of the call lives in `scratch`, one static array per GPU that
+// the library holds until the process ends.  Host code never uses the array's
+// name as an address (it is a host-side stand-in): its device address comes
+// from hipGetSymbolAddress once per context, with the context's device
+// current, and only that address is passed to kernels and copies.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+int ff_noalloc_init(bem_ctx* c) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c->noalloc_ptr = nullptr;
+        return bem_fail(c, BEM_ERR_HIP, std::string("no-allocation scratch: ") + hipGetErrorString(e));
+    }
+    c->noalloc_ptr = p;
+    return 0;
+}
+
+int ff_noalloc_launch(bem_ctx* c, const double* cJ, const double* cM, const double* rh, double kr, double ki,
+                      double eta, int n_calls, int n_orient, int ndir, double* Fv) {
+    if (!c->noalloc_ptr) return bem_fail(c, BEM_ERR_STATE, "farfield_noalloc: no device address of the scratch array");
+    const int N = c->rwg.N;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t bc = up((size_t)n_calls * N * 2 * sizeof(double));       // one interleaved coefficient set
+    const size_t br = up((size_t)n_orient * ndir * 3 * sizeof(double));    // directions
+    const size_t bf = up((size_t)n_calls * ndir * 6 * sizeof(double));     // result
+    const size_t bw = up(farfield_scratch_bytes(c->dm, n_calls, n_orient, ndir));  // currents, partial sums
+    const size_t need = 2 * bc + br + bf + bw;
+    if (need > kNoAllocBytes)
+        return bem_fail(c, BEM_ERR_ARG, "farfield_noalloc: the call needs " + std::to_string(need) +
+                                            " bytes of device staging, above the " + std::to_string(kNoAllocBytes) +
+                                            "-byte no-allocation array (use bem_farfield)");
+    char* base = static_cast<char*>(c->noalloc_ptr);
+    double* dJ = reinterpret_cast<double*>(base);
+    double* dM = reinterpret_cast<double*>(base + bc);
+    double* drh = reinterpret_cast<double*>(base + 2 * bc);
+    double* dFv = reinterpret_cast<double*>(base + 2 * bc + br);
+    char* work = base + 2 * bc + br + bf;
+    hipStream_t s = c->stream;
+    BEM_HIP(c, hipMemcpyAsync(dJ, cJ, (size_t)n_calls * N * 2 * sizeof(double), hipMemcpyHostToDevice, s));
+    BEM_HIP(c, hipMemcpyAsync(dM, cM, (size_t)n_calls * N * 2 * sizeof(double), hipMemcpyHostToDevice, s));
+    BEM_HIP(c, hipMemcpyAsync(drh, rh, (size_t)n_orient * ndir * 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    // the coefficients stay interleaved: call c, RWG n at 2 (c N + n) (+1 imaginary)
+    farfield_core(c, dJ, dJ + 1, dM, dM + 1, 2L * N, 2L, drh, kr, ki, eta, n_calls, n_orient, ndir, dFv, work);
+    BEM_HIP(c, hipGetLastError());
+    BEM_HIP(c, hipMemcpyAsync(Fv, dFv, (size_t)n_calls * ndir * 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+    BEM_HIP(c, hipStreamSynchronize(s));
+    return 0;
+}
+
 int ff_launch(bem_ctx* c, const double* dJr, const double* dJi, const double* dMr, const double* dMi, int ldc,
               const double* drh, double kr, double ki, double eta, int n_calls, int n_orient, int ndir,
               double* dFv) {

@@ -262,6 +262,19 @@ int bem_rhs_planewave(bem_ctx* ctx, double kext_re, double kext_im, double eta_e
  * with call c using orientation c/2, Fv_out n_calls x ndir x 3 complex. */
 int bem_farfield(bem_ctx* ctx, const double* coeffs_J, const double* coeffs_M, const double* r_hats, double kext_re,
                  double kext_im, double eta_ext, int n_calls, int n_orient, int ndir, double* Fv_out);
+/* No counterpart in the reference: bem_farfield for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results as
+ * bem_farfield; it calls no hipMalloc or hipFree: the coefficients,
+ * directions, the currents at the quadrature points, the partial sums and the
+ * result are staged in one static device array of 32 MiB per GPU, held until
+ * the process ends (the far field of one or two orientations fits up to
+ * 2N = 18432 unknowns).  A call that needs more returns BEM_ERR_ARG without
+ * launching anything.  Every call on a GPU shares its array: calls that
+ * overlap on one GPU (from several contexts or threads) race. */
+int bem_farfield_noalloc(bem_ctx* ctx, const double* coeffs_J, const double* coeffs_M, const double* r_hats,
+                         double kext_re, double kext_im, double eta_ext, int n_calls, int n_orient, int ndir,
+                         double* Fv_out);
 /* The dense orientation loop of main.cpp:758-846 on the device solution of
  * bem_solve (column 2o = parallel, 2o+1 = perpendicular polarisation of
  * orientation o): far field at theta[t] in the lab xz-plane rotated by RT
