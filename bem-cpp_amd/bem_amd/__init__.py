@@ -82,6 +82,7 @@ def lib():
         "bem_get_rwg": ([vp, _ip, _ip, _dp, _dp, _dp, _dp, _dp], i),
         "bem_assemble": ([vp, d, d, d, d, d, d, u], i), "bem_get_Z": ([vp, _dp, i], i),
         "bem_get_Z_entries": ([vp, _ip, _ip, i, _dp], i),
+        "bem_near_corrections": ([vp, d, d, d, d, _ip, _ip, _dp, _dp, _dp, _dp], i),
         "bem_assemble_LK": ([vp, d, d, _dp, _dp], i),
         "bem_factor": ([vp, i], i), "bem_set_Z": ([vp, _dp, i, i], i), "bem_get_pivots": ([vp, _ip], i),
         "bem_get_lu": ([vp, _dp], i), "bem_set_lu": ([vp, _dp, _ip, i], i),
@@ -268,6 +269,20 @@ class Context:
         out = np.zeros(r.size, np.complex128)
         self._check(self._L.bem_get_Z_entries(self._h, _pi(r), _pi(cl), r.size, _p(out)), "get_Z_entries")
         return out.reshape(r.shape)
+
+    def near_corrections(self, k_ext: complex, k_int: complex):
+        """Near-field singular corrections of the fast operators
+        (bem_fmm.cu:506-686) for the mesh: CSR (row_ptr, col_idx) and the values
+        of L_ext, K_ext, L_int, K_int (complex128, nnz each)."""
+        k_ext, k_int = complex(k_ext), complex(k_int)
+        args = (self._h, k_ext.real, k_ext.imag, k_int.real, k_int.imag)
+        rp = np.zeros(self.N + 1, np.int32)
+        nnz = self._check(self._L.bem_near_corrections(*args, _pi(rp), None, None, None, None, None),
+                          "near_corrections")
+        ci = np.zeros(nnz, np.int32)
+        vals = [np.zeros(nnz, np.complex128) for _ in range(4)]
+        self._check(self._L.bem_near_corrections(*args, _pi(rp), _pi(ci), *[_p(v) for v in vals]), "near_corrections")
+        return rp, ci, vals
 
     def assemble_LK(self, k: complex):
         k = complex(k)

@@ -583,6 +583,17 @@ int bem_get_Z_entries(bem_ctx* c, const int* rows, const int* cols, int count, d
     return 0;
 }
 
+int bem_near_corrections(bem_ctx* c, double ker, double kei, double kir, double kii, int* row_ptr, int* col_idx,
+                         double* L_ext, double* K_ext, double* L_int, double* K_int) {
+    if (!c || !row_ptr) return bem_fail(c, BEM_ERR_ARG, "near_corrections: bad arguments");
+    if (!c->has_mesh) return bem_fail(c, BEM_ERR_STATE, "near_corrections: no mesh");
+    if ((ker == 0.0 && kei == 0.0) || (kir == 0.0 && kii == 0.0))
+        return bem_fail(c, BEM_ERR_ARG, "near_corrections: zero wavenumber");
+    (void)hipSetDevice(c->device);
+    double* const out[4] = {L_ext, K_ext, L_int, K_int};
+    return near_corr_launch(c, ker, kei, kir, kii, row_ptr, col_idx, out);
+}
+
 int bem_assemble_LK(bem_ctx* c, double kr, double ki, double* L, double* K) {
     if (!c || !L || !K) return BEM_ERR_ARG;
     if (int rc = bem_assemble(c, kr, ki, kr, ki, 1.0, 1.0, BEM_ASM_Z64)) return rc;

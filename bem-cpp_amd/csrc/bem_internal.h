@@ -285,6 +285,12 @@ int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq
 // Gathers count entries Z64[rows[e]][cols[e]] (device index arrays, checked
 // by the caller) into out (device, interleaved complex).
 int z_entries_launch(bem_ctx* c, const int* rows, const int* cols, int count, double* out);
+// Near-field singular corrections of the fast operators (assembly.hip):
+// writes row_ptr (N + 1) and returns nnz; with col_idx also col_idx (nnz) and
+// out[0..3] = L_ext, K_ext, L_int, K_int (nnz interleaved complex each, host;
+// null entries skipped).
+int near_corr_launch(bem_ctx* c, double ker, double kei, double kir, double kii, int* row_ptr, int* col_idx,
+                     double* const out[4]);
 // Bytes of one record array (T (T + 1) / 2 triangle-pair records).
 size_t asm_record_bytes(const bem_ctx* c);
 

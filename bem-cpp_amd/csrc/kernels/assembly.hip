@@ -869,22 +869,9 @@ int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi
 
 }  // namespace
 
-// Record buffer of T(T+1)/2 records, and device copies of the vertex
-// coordinates and quadrature weights used by k_graglia (context scratch).
-static int asm_prepare(bem_ctx* c, const double** dverts, const double** dqw) {
-    const DevMesh& d = c->dm;
-    const size_t T = (size_t)d.T;
-    const size_t need = T * (T + 1) / 2 * kRec * sizeof(double);
-    if (c->H_bytes < need) {
-        if (c->H) (void)hipFree(c->H);
-        c->H = nullptr;
-        c->H_bytes = 0;
-        if (hipMalloc(&c->H, need) != hipSuccess) {
-            c->H = nullptr;
-            return bem_fail(c, BEM_ERR_NOMEM, "assembly scratch: cannot allocate " + std::to_string(need) + " bytes");
-        }
-        c->H_bytes = need;
-    }
+// Device copies of the vertex coordinates and quadrature weights used by
+// k_graglia (context scratch).
+static int asm_prepare_graglia(bem_ctx* c, const double** dverts, const double** dqw) {
     const int nv = c->mesh.nv();
     const size_t sbytes = (size_t)(nv * 3 + 32) * sizeof(double);
     void* s = nullptr;
@@ -900,6 +887,148 @@ static int asm_prepare(bem_ctx* c, const double** dverts, const double** dqw) {
     *dverts = static_cast<const double*>(s);
     *dqw = *dverts + nv * 3;
     return 0;
+}
+
+// Record buffer of T(T+1)/2 records, and the k_graglia inputs.
+static int asm_prepare(bem_ctx* c, const double** dverts, const double** dqw) {
+    const DevMesh& d = c->dm;
+    const size_t T = (size_t)d.T;
+    const size_t need = T * (T + 1) / 2 * kRec * sizeof(double);
+    if (c->H_bytes < need) {
+        if (c->H) (void)hipFree(c->H);
+        c->H = nullptr;
+        c->H_bytes = 0;
+        if (hipMalloc(&c->H, need) != hipSuccess) {
+            c->H = nullptr;
+            return bem_fail(c, BEM_ERR_NOMEM, "assembly scratch: cannot allocate " + std::to_string(need) + " bytes");
+        }
+        c->H_bytes = need;
+    }
+    return asm_prepare_graglia(c, dverts, dqw);
+}
+
+// ---------------------------------------------------------------------------
+// Near-field singular corrections of the fast (FMM / pFFT) operators
+// (BemFmmOperator::precompute_corrections, bem_fmm.cu:506-686), all in FP64.
+// For the halves a, b of two RWG functions on one triangle and wavenumber k:
+//   C_L = ik sum_ij f_a(r_i).f_b(r_j) DG_ij jw_i jw_j
+//         - (i/k) d_a d_b sum_ij DG_ij jw_i jw_j
+//         + ik/(4 pi) sum_i f_a(r_i).c_b (V_i - p_b P_i) jw_i
+//         - (i/k) d_a d_b / (4 pi) sum_i P_i jw_i,
+//   DG = -1/(4 pi R) (R > 1e-12), ik/(4 pi) at R = 0 (the exact limit of the
+//   smooth kernel, where the dense path keeps the clamped i Re(k)/(4 pi));
+//   C_K = -sum_{R > 1e-12} G (ik - 1/R)/R f_a(r_i).((r_i - r_j) x f_b(r_j)) jw_i jw_j,
+// which vanishes on a flat triangle up to rounding (the triple product of
+// three in-plane vectors).  One thread per (triangle, half pair, wavenumber);
+// its sums run in the reference's order.  out[(t * 9 + a * 3 + b) * 2 + w] =
+// (C_L, minus C_K) as four doubles.
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(256) void k_near_corr(int T, const double4* __restrict__ qp,
+                                                   const double* __restrict__ hc, const double* __restrict__ hp,
+                                                   const int* __restrict__ hrwg, const double4* __restrict__ pv,
+                                                   double ker, double kei, double kir, double kii,
+                                                   double* __restrict__ out) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= T * 18) return;
+    const int t = idx / 18, r = idx % 18, pr = r / 2, w = r % 2, a = pr / 3, b = pr % 3;
+    double* o = out + (long)idx * 4;
+    if (hrwg[t * 3 + a] < 0 || hrwg[t * 3 + b] < 0) {
+        o[0] = o[1] = o[2] = o[3] = 0.0;
+        return;
+    }
+    const double kr = w ? kir : ker, ki = w ? kii : kei;
+    const double ca = hc[t * 3 + a], cb = hc[t * 3 + b];
+    const double pa[3] = {hp[(t * 3 + a) * 3], hp[(t * 3 + a) * 3 + 1], hp[(t * 3 + a) * 3 + 2]};
+    const double pb[3] = {hp[(t * 3 + b) * 3], hp[(t * 3 + b) * 3 + 1], hp[(t * 3 + b) * 3 + 2]};
+    const double da = 2.0 * ca, db = 2.0 * cb;
+    // ik = (-ki, kr); i/k = (ki + i kr) / |k|^2
+    const double ikr = -ki, iki = kr;
+    const double k2 = kr * kr + ki * ki, iokr = ki / k2, ioki = kr / k2;
+    double dvr = 0, dvi = 0, dsr = 0, dsi = 0, kcr = 0, kci = 0;
+    for (int i = 0; i < NQ; i++) {
+        const double4 qi = qp[t * NQ + i];
+        const double fa[3] = {ca * (qi.x - pa[0]), ca * (qi.y - pa[1]), ca * (qi.z - pa[2])};
+        for (int j = 0; j < NQ; j++) {
+            const double4 qj = qp[t * NQ + j];
+            const double dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+            const double R = sqrt(dx * dx + dy * dy + dz * dz);
+            const double jp = qi.w * qj.w;
+            const double fb[3] = {cb * (qj.x - pb[0]), cb * (qj.y - pb[1]), cb * (qj.z - pb[2])};
+            const double fd = fa[0] * fb[0] + fa[1] * fb[1] + fa[2] * fb[2];
+            double gr, gi;  // DG
+            if (R > 1e-12) {
+                gr = -1.0 / (4.0 * kPi * R);
+                gi = 0.0;
+            } else {
+                gr = ikr / (4.0 * kPi);
+                gi = iki / (4.0 * kPi);
+            }
+            // dv += ik * fd * DG * jp; ds += -iok * da * db * DG * jp
+            const double tr = ikr * fd, ti = iki * fd;
+            dvr += (tr * gr - ti * gi) * jp;
+            dvi += (tr * gi + ti * gr) * jp;
+            const double sr = -iokr * da * db, si = -ioki * da * db;
+            dsr += (sr * gr - si * gi) * jp;
+            dsi += (sr * gi + si * gr) * jp;
+            if (R >= 1e-12) {
+                // G = exp(ikR) / (4 pi R), exp(ikR) = exp(-ki R) (cos kr R + i sin kr R)
+                double sn, cs;
+                sincos(kr * R, &sn, &cs);
+                const double e = exp(-ki * R) / (4.0 * kPi * R);
+                const double Gr = e * cs, Gi = e * sn;
+                const double hr = (ikr - 1.0 / R) / R, hi = iki / R;  // (ik - 1/R) / R
+                const double ggr = Gr * hr - Gi * hi, ggi = Gr * hi + Gi * hr;
+                const double cx = dy * fb[2] - dz * fb[1], cy = dz * fb[0] - dx * fb[2], cz = dx * fb[1] - dy * fb[0];
+                const double tp = (fa[0] * cx + fa[1] * cy + fa[2] * cz) * jp;
+                kcr += ggr * tp;
+                kci += ggi * tp;
+            }
+        }
+    }
+    double avr = 0, avi = 0, asr = 0, asi = 0;
+    for (int i = 0; i < NQ; i++) {
+        const double4 qi = qp[t * NQ + i];
+        const double4 g = pv[t * NQ + i];  // P, V
+        const double fa[3] = {ca * (qi.x - pa[0]), ca * (qi.y - pa[1]), ca * (qi.z - pa[2])};
+        const double fo[3] = {cb * (g.y - pb[0] * g.x), cb * (g.z - pb[1] * g.x), cb * (g.w - pb[2] * g.x)};
+        const double fdo = fa[0] * fo[0] + fa[1] * fo[1] + fa[2] * fo[2];
+        avr += ikr * fdo * qi.w * kInv4Pi;
+        avi += iki * fdo * qi.w * kInv4Pi;
+        asr += -iokr * da * db * g.x * qi.w * kInv4Pi;
+        asi += -ioki * da * db * g.x * qi.w * kInv4Pi;
+    }
+    o[0] = ((dvr + dsr) + avr) + asr;
+    o[1] = ((dvi + dsi) + avi) + asi;
+    o[2] = kcr;
+    o[3] = kci;
+}
+
+// CSR values from the per-triangle contributions: entry e takes the slots
+// src[2e] and src[2e + 1] (-1: none) in that (triangle) order.
+__global__ __launch_bounds__(256) void k_near_gather(int nnz, const int* __restrict__ src,
+                                                     const double* __restrict__ contrib, double* __restrict__ vals) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    for (int w = 0; w < 2; w++) {
+        double lr = 0, li = 0, kr = 0, ki = 0;
+        for (int u = 0; u < 2; u++) {
+            const int sl = src[2 * e + u];
+            if (sl < 0) continue;
+            const double* c = contrib + ((long)sl * 2 + w) * 4;
+            lr += c[0];
+            li += c[1];
+            kr -= c[2];
+            ki -= c[3];
+        }
+        // vals: L_ext, K_ext, L_int, K_int, each nnz interleaved complex
+        double* L = vals + (long)(2 * w) * 2 * nnz;
+        double* K = vals + (long)(2 * w + 1) * 2 * nnz;
+        L[2 * e] = lr;
+        L[2 * e + 1] = li;
+        K[2 * e] = kr;
+        K[2 * e + 1] = ki;
+    }
 }
 
 int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
@@ -920,6 +1049,95 @@ int asm_launch(bem_ctx* c, const AsmParams& p, bool want64, bool want32) {
     BEM_HIP(c, hipGetLastError());
     BEM_HIP(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int near_corr_launch(bem_ctx* c, double ker, double kei, double kir, double kii, int* row_ptr, int* col_idx,
+                     double* const out[4]) {
+    const DevMesh& d = c->dm;
+    const int T = d.T, N = d.N;
+    // CSR pattern (bem_fmm.cu:540-567) and, per entry, its contributing
+    // (triangle, half a, half b) slots in triangle order
+    std::vector<int> hr((size_t)T * 3);
+    BEM_HIP(c, hipMemcpy(hr.data(), d.hrwg, hr.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<std::vector<std::pair<int, int>>> rows(N);  // (column, slot)
+    for (int t = 0; t < T; t++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                const int m = hr[t * 3 + a], n = hr[t * 3 + b];
+                if (m >= 0 && n >= 0) rows[m].push_back({n, t * 9 + a * 3 + b});
+            }
+    std::vector<int> rp(N + 1, 0), ci, src;
+    for (int m = 0; m < N; m++) {
+        std::stable_sort(rows[m].begin(), rows[m].end(),
+                         [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
+        for (size_t i = 0; i < rows[m].size();) {
+            size_t j = i;
+            int s0 = -1, s1 = -1;
+            while (j < rows[m].size() && rows[m][j].first == rows[m][i].first) {
+                if (s0 < 0)
+                    s0 = rows[m][j].second;
+                else if (s1 < 0)
+                    s1 = rows[m][j].second;
+                else
+                    return bem_fail(c, BEM_ERR_MESH, "near corrections: a basis pair shares more than two triangles");
+                j++;
+            }
+            ci.push_back(rows[m][i].first);
+            src.push_back(s0);
+            src.push_back(s1);
+            i = j;
+        }
+        rp[m + 1] = (int)ci.size();
+    }
+    const int nnz = rp[N];
+    if (row_ptr) std::copy(rp.begin(), rp.end(), row_ptr);
+    if (!col_idx) return nnz;
+    std::copy(ci.begin(), ci.end(), col_idx);
+    if (nnz == 0) return 0;
+    const double *dverts, *dqw;
+    if (int rc = asm_prepare_graglia(c, &dverts, &dqw)) return rc;
+    // device: contributions (T * 18 * 4), sources (2 nnz ints), values (8 nnz)
+    const size_t bc = (size_t)T * 18 * 4 * sizeof(double), bs = ((size_t)2 * nnz * sizeof(int) + 255) / 256 * 256;
+    const size_t bv = (size_t)8 * nnz * sizeof(double);
+    char* w = nullptr;
+    if (hipMalloc(&w, bc + bs + bv) != hipSuccess) return bem_fail(c, BEM_ERR_NOMEM, "near corrections: work area");
+    double* contrib = reinterpret_cast<double*>(w);
+    int* dsrc = reinterpret_cast<int*>(w + bc);
+    double* dvals = reinterpret_cast<double*>(w + bc + bs);
+    hipError_t e = hipMemcpyAsync(dsrc, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const dim3 g1((unsigned)((T * 18 + 255) / 256)), g2((unsigned)((nnz + 255) / 256));
+        switch (d.Nq) {
+            case 4:
+                hipLaunchKernelGGL(k_graglia<4>, dim3((T * 4 + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv,
+                                   dverts, dqw, d.pv);
+                hipLaunchKernelGGL(k_near_corr<4>, g1, dim3(256), 0, c->stream, T, d.qp, d.hc, d.hp, d.hrwg, d.pv, ker,
+                                   kei, kir, kii, contrib);
+                break;
+            case 13:
+                hipLaunchKernelGGL(k_graglia<13>, dim3((T * 13 + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv,
+                                   dverts, dqw, d.pv);
+                hipLaunchKernelGGL(k_near_corr<13>, g1, dim3(256), 0, c->stream, T, d.qp, d.hc, d.hp, d.hrwg, d.pv,
+                                   ker, kei, kir, kii, contrib);
+                break;
+            default:
+                hipLaunchKernelGGL(k_graglia<7>, dim3((T * 7 + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv,
+                                   dverts, dqw, d.pv);
+                hipLaunchKernelGGL(k_near_corr<7>, g1, dim3(256), 0, c->stream, T, d.qp, d.hc, d.hp, d.hrwg, d.pv, ker,
+                                   kei, kir, kii, contrib);
+                break;
+        }
+        hipLaunchKernelGGL(k_near_gather, g2, dim3(256), 0, c->stream, nnz, dsrc, contrib, dvals);
+        e = hipGetLastError();
+    }
+    std::vector<double> hv((size_t)8 * nnz);
+    if (e == hipSuccess) e = hipMemcpyAsync(hv.data(), dvals, bv, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(w);
+    if (e != hipSuccess) return bem_fail(c, BEM_ERR_HIP, std::string("near corrections: ") + hipGetErrorString(e));
+    for (int a = 0; a < 4; a++)
+        if (out[a]) std::copy(hv.begin() + (size_t)a * 2 * nnz, hv.begin() + (size_t)(a + 1) * 2 * nnz, out[a]);
+    return nnz;
 }
 
 namespace {

@@ -9,13 +9,21 @@
 // MFMA steps of 4), so the choice changes speed only, not results.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <map>
 #include <mutex>
+#include <string>
 #include <tuple>
+#include <vector>
 
 #include "../bem_internal.h"
 #include "gemm.h"
+#include "gemm_nt.h"
 
 namespace {
 
@@ -77,6 +85,172 @@ int time_configs(hipStream_t s, const Shape& sh, T* buf, double* rates, const Ge
 std::mutex g_mu;
 std::map<std::tuple<int, int, int, int>, GemmPlan> g_cache;
 
+// ---------------------------------------------------------------------------
+// Plans persisted across processes: a JSON file (BEM_TUNE_CACHE, else
+// $XDG_CACHE_HOME/bem_amd/gemm_plans.json, else ~/.cache/bem_amd/..., else,
+// without a writable home, /tmp/bem_amd-<uid>/bem_amd/...; an empty
+// BEM_TUNE_CACHE turns the file off), one plan per line, keyed by the device
+// (architecture name and CU count), the precision, npad and ldr.  A plan found
+// there replaces the timing of the tiles and, when it was chosen by trial
+// factorisations, the trials too.  kPlanFormat changes whenever the tile
+// configurations change, so plans of another build are ignored.
+// ---------------------------------------------------------------------------
+constexpr int kPlanFormat = 2;
+
+std::string plan_file() {
+    if (const char* e = std::getenv("BEM_TUNE_CACHE")) return e;
+    std::string base;
+    if (const char* x = std::getenv("XDG_CACHE_HOME"); x && *x && ::access(x, W_OK) == 0)
+        base = x;
+    else if (const char* h = std::getenv("HOME"); h && *h && ::access(h, W_OK) == 0)
+        base = std::string(h) + "/.cache";
+    else  // no writable home (service accounts): a per-user folder in /tmp
+        base = "/tmp/bem_amd-" + std::to_string((long)::getuid());
+    return base + "/bem_amd/gemm_plans.json";
+}
+
+std::string device_tag(int device) {
+    hipDeviceProp_t p{};
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
+    return std::string(p.gcnArchName) + "/" + std::to_string(p.multiProcessorCount);
+}
+
+std::string plan_line(const std::string& dev, int prec, int npad, int ldr, const GemmPlan& p) {
+    char b[1024];
+    int k = std::snprintf(b, sizeof b,
+                          "{\"format\": %d, \"device\": \"%s\", \"prec\": %d, \"npad\": %d, \"ldr\": %d, "
+                          "\"large\": %d, \"skinny\": %d, \"panel\": %d, \"nt\": %d, \"fill\": %ld, "
+                          "\"trialed\": %d, \"trial_ms\": [",
+                          kPlanFormat, dev.c_str(), prec, npad, ldr, p.large, p.skinny, p.panel, p.nt, p.fill,
+                          p.trialed ? 1 : 0);
+    for (int i = 0; i < GemmPlan::kMaxCfgs; i++)
+        k += std::snprintf(b + k, sizeof b - k, "%s%.17g", i ? ", " : "", p.trial_ms[i]);
+    k += std::snprintf(b + k, sizeof b - k, "], \"alone_tflops\": [");
+    for (int i = 0; i < GemmPlan::kMaxCfgs; i++)
+        k += std::snprintf(b + k, sizeof b - k, "%s%.17g", i ? ", " : "", p.alone_tflops[i]);
+    std::snprintf(b + k, sizeof b - k, "]}");
+    return b;
+}
+
+// Number after "key": in line (false when absent).
+bool json_num(const std::string& line, const char* key, double& v) {
+    const std::string k = std::string("\"") + key + "\":";
+    const size_t at = line.find(k);
+    if (at == std::string::npos) return false;
+    char* end = nullptr;
+    v = std::strtod(line.c_str() + at + k.size(), &end);
+    return end != line.c_str() + at + k.size();
+}
+bool json_array(const std::string& line, const char* key, double* v, int n) {
+    const std::string k = std::string("\"") + key + "\": [";
+    size_t at = line.find(k);
+    if (at == std::string::npos) return false;
+    const char* q = line.c_str() + at + k.size();
+    for (int i = 0; i < n; i++) {
+        char* end = nullptr;
+        v[i] = std::strtod(q, &end);
+        if (end == q) return false;
+        q = end;
+        while (*q == ',' || *q == ' ') q++;
+    }
+    return true;
+}
+bool json_str(const std::string& line, const char* key, std::string& v) {
+    const std::string k = std::string("\"") + key + "\": \"";
+    const size_t at = line.find(k);
+    if (at == std::string::npos) return false;
+    const size_t e = line.find('"', at + k.size());
+    if (e == std::string::npos) return false;
+    v = line.substr(at + k.size(), e - (at + k.size()));
+    return true;
+}
+
+std::vector<std::string> plan_file_lines(const std::string& path) {
+    std::vector<std::string> out;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line))
+        if (line.rfind("{\"format\"", 0) == 0) out.push_back(line);
+    return out;
+}
+
+// The persisted plan of (device, prec, npad, ldr) into p; false if none.
+bool plan_load(int device, int prec, int npad, int ldr, GemmPlan& p) {
+    const std::string path = plan_file();
+    if (path.empty()) return false;
+    const std::string dev = device_tag(device);
+    for (const std::string& line : plan_file_lines(path)) {
+        double fmt = 0, pr = 0, np = 0, lr = 0, v = 0;
+        std::string d;
+        if (!json_num(line, "format", fmt) || (int)fmt != kPlanFormat || !json_str(line, "device", d) || d != dev ||
+            !json_num(line, "prec", pr) || (int)pr != prec || !json_num(line, "npad", np) || (int)np != npad ||
+            !json_num(line, "ldr", lr) || (int)lr != ldr)
+            continue;
+        GemmPlan q = p;
+        if (!json_num(line, "large", v)) continue;
+        q.large = (int)v;
+        if (!json_num(line, "skinny", v)) continue;
+        q.skinny = (int)v;
+        if (!json_num(line, "panel", v)) continue;
+        q.panel = (int)v;
+        if (!json_num(line, "nt", v)) continue;
+        q.nt = (int)v;
+        if (!json_num(line, "fill", v)) continue;
+        q.fill = (long)v;
+        if (!json_num(line, "trialed", v)) continue;
+        q.trialed = v != 0.0;
+        if (!json_array(line, "trial_ms", q.trial_ms, GemmPlan::kMaxCfgs)) continue;
+        if (!json_array(line, "alone_tflops", q.alone_tflops, GemmPlan::kMaxCfgs)) continue;
+        if (q.large < 0 || q.large >= kGemmCfgs || q.skinny < 0 || q.skinny >= kGemmCfgs || q.panel < 0 ||
+            q.panel >= kGemmCfgs || q.nt < 0 || q.nt >= kNtCfgs)
+            continue;
+        q.tuned = true;
+        q.npad = npad;
+        q.cap = 0;
+        p = q;
+        return true;
+    }
+    return false;
+}
+
+// Writes (or replaces) the plan's line: the file is rewritten through a
+// temporary file and a rename, so a reader never sees half a file.
+void plan_save(int device, int prec, int npad, int ldr, const GemmPlan& p) {
+    const std::string path = plan_file();
+    if (path.empty()) return;
+    const std::string dev = device_tag(device);
+    std::vector<std::string> keep;
+    for (const std::string& line : plan_file_lines(path)) {
+        double fmt = 0, pr = 0, np = 0, lr = 0;
+        std::string d;
+        const bool same = json_num(line, "format", fmt) && (int)fmt == kPlanFormat && json_str(line, "device", d) &&
+                          d == dev && json_num(line, "prec", pr) && (int)pr == prec && json_num(line, "npad", np) &&
+                          (int)np == npad && json_num(line, "ldr", lr) && (int)lr == ldr;
+        if (!same) keep.push_back(line);
+    }
+    keep.push_back(plan_line(dev, prec, npad, ldr, p));
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos) {  // create the directory chain
+        const std::string dir = path.substr(0, slash);
+        for (size_t i = 1; i <= dir.size(); i++)
+            if (i == dir.size() || dir[i] == '/') (void)::mkdir(dir.substr(0, i).c_str(), 0755);
+    }
+    const std::string tmp = path + ".tmp" + std::to_string((long)::getpid());
+    {
+        std::ofstream f(tmp);
+        if (!f) return;
+        f << "{\"bem_amd_gemm_plans\": " << kPlanFormat << ", \"plans\": [\n";
+        for (size_t i = 0; i < keep.size(); i++) f << keep[i] << (i + 1 < keep.size() ? ",\n" : "\n");
+        f << "]}\n";
+        if (!f) {
+            f.close();
+            (void)std::remove(tmp.c_str());
+            return;
+        }
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) (void)std::remove(tmp.c_str());
+}
+
 template <typename T>
 int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* rates_skinny) {
     GemmPlan& plan = c->gemm_plan[sizeof(T) == 4 ? 0 : 1];
@@ -86,6 +260,11 @@ int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* r
         auto it = g_cache.find(key);
         if (it != g_cache.end() && !rates_large) {
             plan = it->second;  // carries npad
+            return 0;
+        }
+        if (!rates_large && plan_load(c->device, (int)sizeof(T), npad, ldr, plan)) {
+            gemm_plan_occupancy(c, sizeof(T) == 4);
+            g_cache[key] = plan;
             return 0;
         }
     }
@@ -123,6 +302,7 @@ int tune_precision(bem_ctx* c, int npad, int ldr, double* rates_large, double* r
         }
     std::lock_guard<std::mutex> lk(g_mu);
     g_cache[key] = plan;
+    plan_save(c->device, (int)sizeof(T), npad, ldr, plan);
     return 0;
 }
 
@@ -162,6 +342,7 @@ void gemm_cache_store(bem_ctx* c, bool f32) {
     const int ldr = round_up(std::max(c->ldr > 0 ? c->ldr : 128, 1), 16);
     std::lock_guard<std::mutex> lk(g_mu);
     g_cache[std::make_tuple(c->device, f32 ? 4 : 8, plan.npad, ldr)] = plan;
+    plan_save(c->device, f32 ? 4 : 8, plan.npad, ldr, plan);
 }
 
 // Fixes the plan of one precision (0 FP32, 1 FP64) instead of tuning it.

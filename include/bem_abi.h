@@ -176,6 +176,19 @@ int bem_get_Z(bem_ctx* ctx, double* Z, int row_major);
  * out[2e], out[2e+1] (interleaved complex).  No reference counterpart: the
  * sampled form of bem_get_Z, for checks of large systems without copying Z. */
 int bem_get_Z_entries(bem_ctx* ctx, const int* rows, const int* cols, int count, double* out);
+/* Replaces BemFmmOperator::precompute_corrections (bem_fmm.h:165-166,
+ * bem_fmm.cu:506-686): the near-field singular corrections that the fast (FMM
+ * / pFFT) operators add to their point sums, for the context's mesh and
+ * quadrature and the wavenumbers k_ext, k_int.  For every pair (m, n) of RWG
+ * functions with halves on one triangle: C_L = the smooth same-triangle
+ * quadrature (R = 0 taken at its exact limit ik/(4 pi)) plus the Graglia
+ * static part minus the point sum of the full kernel, C_K = minus the point
+ * sum's K term.  Output in the reference's CSR form (bem_fmm.h:56-64):
+ * row_ptr (N + 1), col_idx (nnz, sorted per row), and the values of L_ext,
+ * K_ext, L_int, K_int (nnz interleaved complex each; any may be NULL).  With
+ * col_idx == NULL only row_ptr is written.  Returns nnz, or < 0. */
+int bem_near_corrections(bem_ctx* ctx, double kext_re, double kext_im, double kint_re, double kint_im, int* row_ptr,
+                         int* col_idx, double* L_ext, double* K_ext, double* L_int, double* K_int);
 /* Replaces assemble_L_K_cuda (assembly.h:11-13): L and K (N x N row-major) for
  * one wavenumber.  Runs the same kernels with k_ext = k_int = k, eta = 1. */
 int bem_assemble_LK(bem_ctx* ctx, double k_re, double k_im, double* L, double* K);

@@ -862,6 +862,122 @@ static int write_json(const char* fn, const double* M, const double* th, int nt,
 // extern "C" surface for the Python tests (ctypes).  Complex arrays are
 // interleaved (re, im) doubles.
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// Near-field singular corrections of the fast (FMM / pFFT) operators:
+// BemFmmOperator::precompute_corrections, bem_fmm.cu:506-686 (FP64, as there).
+// For every pair (m, n) of RWG functions with halves on one triangle, per
+// wavenumber: C_L = ik sum f.f DG jw jw - (i/k) d d sum DG jw jw + the
+// Graglia static part, with DG = -1/(4 pi R) at R > 1e-12 and the exact R = 0
+// limit ik/(4 pi) (bem_fmm.cu:603-615); C_K = -sum gradG (f_m . (r_i - r_j) x
+// f_n) jw jw over R > 1e-12 (bem_fmm.cu:664-680).  CSR rows sorted by column
+// (bem_fmm.cu:540-567); values summed over the shared triangles in triangle
+// order (the std::map of bem_fmm.cu:521-537).
+// ---------------------------------------------------------------------------
+struct NearCorr {
+    std::vector<int> row_ptr, col;
+    std::vector<cplx> L[2], K[2];  // [0] k_ext, [1] k_int
+};
+
+static NearCorr near_corrections(const Rwg& r, const MeshD& m, const Quad& q, cplx ke, cplx ki) {
+    const int N = r.N, Nq = q.n;
+    struct Hi { int n; double div, coeff; V3 fv; int sign; double area; };
+    std::map<int, std::vector<Hi>> t2r;  // bem_fmm.cu:521-537
+    for (int n = 0; n < N; n++) {
+        t2r[r.tp[n]].push_back({n, r.len[n] / r.ap[n], r.len[n] / (2.0 * r.ap[n]), r.fp[n], +1, r.ap[n]});
+        t2r[r.tm[n]].push_back({n, -r.len[n] / r.am[n], r.len[n] / (2.0 * r.am[n]), r.fm[n], -1, r.am[n]});
+    }
+    NearCorr C;
+    std::vector<std::vector<int>> rows(N);  // bem_fmm.cu:540-555
+    for (auto& kv : t2r)
+        for (const Hi& a : kv.second)
+            for (const Hi& b : kv.second) rows[a.n].push_back(b.n);
+    C.row_ptr.assign(N + 1, 0);
+    for (int i = 0; i < N; i++) {
+        std::sort(rows[i].begin(), rows[i].end());
+        rows[i].erase(std::unique(rows[i].begin(), rows[i].end()), rows[i].end());
+        C.row_ptr[i + 1] = C.row_ptr[i] + (int)rows[i].size();
+    }
+    for (int i = 0; i < N; i++) C.col.insert(C.col.end(), rows[i].begin(), rows[i].end());
+    const int nnz = C.row_ptr[N];
+    for (int w = 0; w < 2; w++) {
+        C.L[w].assign(nnz, cplx(0.0));
+        C.K[w].assign(nnz, cplx(0.0));
+    }
+    const cplx I(0.0, 1.0);
+    for (auto& kv : t2r) {  // bem_fmm.cu:573-686
+        const std::vector<Hi>& hs = kv.second;
+        V3 v0, v1, v2;
+        m.tri(kv.first, v0, v1, v2);
+        std::vector<V3> qp(Nq), V(Nq);
+        std::vector<double> P(Nq), R((size_t)Nq * Nq);
+        for (int i = 0; i < Nq; i++) {
+            const double l0 = 1.0 - q.l1[i] - q.l2[i];
+            qp[i] = v0 * l0 + v1 * q.l1[i] + v2 * q.l2[i];
+        }
+        for (int i = 0; i < Nq; i++)
+            for (int j = 0; j < Nq; j++) R[(size_t)i * Nq + j] = (qp[i] - qp[j]).norm();
+        for (int i = 0; i < Nq; i++) {
+            P[i] = pot_integral(qp[i], v0, v1, v2);
+            V[i] = vec_pot_integral(qp[i], v0, v1, v2, q);
+        }
+        // f and jw of each half on this triangle (bem_fmm.cu: f_p / f_m, jw_p / jw_m)
+        std::vector<std::vector<V3>> f(hs.size(), std::vector<V3>(Nq));
+        std::vector<std::vector<double>> jw(hs.size(), std::vector<double>(Nq));
+        for (size_t h = 0; h < hs.size(); h++)
+            for (int i = 0; i < Nq; i++) {
+                f[h][i] = (qp[i] - hs[h].fv) * (hs[h].sign * hs[h].coeff);
+                jw[h][i] = hs[h].area * q.w[i];
+            }
+        const cplx kv2[2] = {ke, ki};
+        for (int w = 0; w < 2; w++) {
+            const cplx k = kv2[w], ik = I * k, iok = I / k;
+            std::vector<cplx> DG((size_t)Nq * Nq), gG((size_t)Nq * Nq, cplx(0.0));
+            for (size_t e = 0; e < DG.size(); e++) {
+                const double Rij = R[e];
+                if (Rij > 1e-12) {
+                    DG[e] = -1.0 / (4.0 * PI * Rij);
+                    const cplx G = std::exp(ik * Rij) / (4.0 * PI * Rij);
+                    gG[e] = G * (ik - 1.0 / Rij) / Rij;
+                } else {
+                    DG[e] = ik / (4.0 * PI);
+                }
+            }
+            for (size_t a = 0; a < hs.size(); a++)
+                for (size_t b = 0; b < hs.size(); b++) {
+                    const int mrow = hs[a].n, ncol = hs[b].n;
+                    const int* cb = C.col.data() + C.row_ptr[mrow];
+                    const int* ce = C.col.data() + C.row_ptr[mrow + 1];
+                    const int pos = C.row_ptr[mrow] + (int)(std::lower_bound(cb, ce, ncol) - cb);
+                    cplx dv = 0.0, ds = 0.0;
+                    for (int i = 0; i < Nq; i++)
+                        for (int j = 0; j < Nq; j++) {
+                            const double jp = jw[a][i] * jw[b][j];
+                            const double fd = f[a][i].dot(f[b][j]);
+                            dv += ik * fd * DG[(size_t)i * Nq + j] * jp;
+                            ds += -iok * hs[a].div * hs[b].div * DG[(size_t)i * Nq + j] * jp;
+                        }
+                    cplx av = 0.0, as = 0.0;
+                    for (int i = 0; i < Nq; i++) {
+                        const V3 fo = (V[i] - hs[b].fv * P[i]) * (hs[b].sign * hs[b].coeff);
+                        av += ik * f[a][i].dot(fo) * jw[a][i] * INV4PI;
+                        as += -iok * hs[a].div * hs[b].div * P[i] * jw[a][i] * INV4PI;
+                    }
+                    C.L[w][pos] += dv + ds + av + as;
+                    cplx kc = 0.0;
+                    for (int i = 0; i < Nq; i++)
+                        for (int j = 0; j < Nq; j++) {
+                            if (R[(size_t)i * Nq + j] < 1e-12) continue;
+                            const double jp = jw[a][i] * jw[b][j];
+                            const V3 cr = (qp[i] - qp[j]).cross(f[b][j]);
+                            kc += gG[(size_t)i * Nq + j] * f[a][i].dot(cr) * jp;
+                        }
+                    C.K[w][pos] -= kc;
+                }
+        }
+    }
+    return C;
+}
+
 struct orc_problem {
     MeshD mesh;
     Rwg rwg;
@@ -1151,6 +1267,24 @@ int orc_dense_orient(orc_problem* p, double ka, double mre, double mim, int na, 
     p->last_t_solve = t2 - t1;
     p->last_t_farfield = t3 - t2;
     return info;
+}
+
+// Near-field corrections (bem_fmm.cu:506-686) for k_ext and k_int: returns
+// nnz; with col_idx non-null also fills row_ptr (N + 1), col_idx (nnz) and the
+// four value arrays (nnz interleaved complex each).
+int orc_near_corrections(orc_problem* p, double ker, double kei, double kir, double kii, int* row_ptr, int* col_idx,
+                         double* Lext, double* Kext, double* Lint, double* Kint) {
+    NearCorr C = near_corrections(p->rwg, p->mesh, p->quad, cplx(ker, kei), cplx(kir, kii));
+    const int nnz = C.row_ptr.back();
+    if (row_ptr) std::memcpy(row_ptr, C.row_ptr.data(), C.row_ptr.size() * sizeof(int));
+    if (col_idx) {
+        std::memcpy(col_idx, C.col.data(), nnz * sizeof(int));
+        double* out[4] = {Lext, Kext, Lint, Kint};
+        const std::vector<cplx>* src[4] = {&C.L[0], &C.K[0], &C.L[1], &C.K[1]};
+        for (int a = 0; a < 4; a++)
+            if (out[a]) std::memcpy(out[a], src[a]->data(), nnz * sizeof(cplx));
+    }
+    return nnz;
 }
 
 void orc_last_timings(const orc_problem* p, double* out3) {

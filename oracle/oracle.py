@@ -83,6 +83,8 @@ def lib():
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_r0_exact.argtypes = [ctypes.c_int]
+        L.orc_near_corrections.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_ip, _ip] + [_dp] * 4
+        L.orc_near_corrections.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -248,6 +250,19 @@ class Problem:
         info = lib().orc_dense_orient(self._h, ka, m.real, m.imag, na, nb, ng, beta_sym, gamma_sym, int(mirror),
                                       i0, i1, ntheta, _p(M), _p(X))
         return info, M, X.T.copy()
+
+    def near_corrections(self, k_ext: complex, k_int: complex):
+        """Near-field singular corrections of the fast operators
+        (bem_fmm.cu:506-686): CSR (row_ptr, col_idx) and the values of
+        L_ext, K_ext, L_int, K_int (complex128, nnz each)."""
+        k_ext, k_int = complex(k_ext), complex(k_int)
+        args = (self._h, k_ext.real, k_ext.imag, k_int.real, k_int.imag)
+        rp = np.zeros(self.N + 1, np.int32)
+        nnz = lib().orc_near_corrections(*args, _pi(rp), None, None, None, None, None)
+        ci = np.zeros(nnz, np.int32)
+        vals = [np.zeros(nnz, np.complex128) for _ in range(4)]
+        lib().orc_near_corrections(*args, _pi(rp), _pi(ci), *[_p(v) for v in vals])
+        return rp, ci, vals
 
     def last_timings(self):
         out = np.zeros(3)

@@ -424,3 +424,42 @@ def test_json_writer_format(oracle, tmp_path):
     assert d["theta"][-1] == 180.0
     assert np.allclose(np.array(d["mueller"]), M, rtol=1e-8)
     assert '"ka": 5.000000,' in txt and "1.23450000e-03" in txt
+
+
+# ---------------------------------------------------------------- near-field corrections (fast operators)
+def _csr_dense(N, rp, ci, v):
+    D = np.zeros((N, N), np.complex128)
+    for i in range(N):
+        D[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    return D
+
+
+@pytest.mark.parametrize("tag", ["nearcorr_ico0", "nearcorr_hex0"])
+def test_near_corrections_golden(oracle, tag):
+    # BemFmmOperator::precompute_corrections (bem_fmm.cu:506-686) against the
+    # reference's Python twin (bem_fmm.py:1193-1324) on ref-0 meshes
+    g = golden(f"{tag}.npz")
+    p = oracle.Problem(g["verts"], g["tris"])
+    rp, ci, vals = p.near_corrections(complex(g["k_ext"]), complex(g["k_int"]))
+    order = python_to_sorted(g["keys"])
+    N = p.N
+    for name, v in zip(["L_ext", "L_int"], [vals[0], vals[2]]):
+        D = _csr_dense(N, rp, ci, v)
+        G = g[name][np.ix_(order, order)]
+        # the pattern: RWG pairs with halves on one triangle (bem_fmm.cu:540-555)
+        assert np.array_equal(D != 0, G != 0), name
+        # the Graglia V (central difference with step 1e-6) limits the agreement
+        assert np.abs(D - G).max() <= 1e-9 * np.abs(G).max(), (name, np.abs(D - G).max())
+    # every row holds itself and its neighbours through both triangles, sorted
+    for i in range(N):
+        row = ci[rp[i]:rp[i + 1]]
+        assert i in row and np.all(np.diff(row) > 0)
+
+
+def test_near_corrections_k_vanishes_on_flat_triangles(oracle):
+    # f_m(r_i) . ((r_i - r_j) x f_n(r_j)) = 0 for three vectors in one plane:
+    # the K correction of a flat triangle is rounding only
+    p = oracle.Problem.from_shape("hex", 1, 1.4286)
+    rp, ci, vals = p.near_corrections(2.0 + 0j, 2.0 * (1.33 + 0.01j))
+    scale = max(np.abs(vals[0]).max(), np.abs(vals[2]).max())
+    assert np.abs(vals[1]).max() <= 1e-14 * scale and np.abs(vals[3]).max() <= 1e-14 * scale
