@@ -362,7 +362,9 @@ class Context:
 
     def block_jacobi(self, radius_mult=2.0, max_block_rwg=1500, overlap_layers=1):
         """Builds the block-Jacobi / RAS preconditioner from the device's FP64 Z
-        (precond.cu:407-895); returns (block count, build ms)."""
+        (precond.cu:407-895); returns (block count, build ms).  The build time
+        grows with the cube of the block size: for large meshes take the
+        reference's automatic parameters, auto_precond_params(N, ka)."""
         t = ctypes.c_double()
         nb = self._check(self._L.bem_block_jacobi(self._h, float(radius_mult), int(max_block_rwg),
                                                   int(overlap_layers), ctypes.byref(t)), "block_jacobi")

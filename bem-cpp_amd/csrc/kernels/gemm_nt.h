@@ -5,8 +5,8 @@
 // planar complex FP32 in the 3M form (T1 = Ar Br, T2 = Ai Bi,
 // T3 = (Ar + Ai)(Br + Bi); Re = T1 - T2, Im = T3 - T1 - T2), with both
 // operands k-contiguous: A is the panel L21 as it lies in the matrix, Bt the
-// transpose of U12 (written by k_transpose_u12 beside the copy of U12 into
-// the matrix).
+// transpose of U12 (written by k_u12_out beside the copy of U12 into the
+// matrix).
 //
 // Why both operands k-contiguous: v_mfma_f32_16x16x4f32 takes one k per lane
 // group g = lane / 16.  Four consecutive MFMAs s = 0..3 use k = 4g + s, so a

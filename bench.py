@@ -166,7 +166,14 @@ def cpu_baseline(shape, ref, param, ka, m, n_orient, ntheta, N_full, full=False)
                   f"{lapack} factor + {2 * n_orient}-RHS solve of a random n={n_s} system x (2N/{n_s})^3; "
                   f"extrapolated to N={N_full}: assembly {t_asm:.3g} s, LU+solve {t_lu:.3g} s, far field "
                   f"{t_ff_per_orient * n_orient:.3g} s")
-    return {"value": total / n_orient, "unit": "s/orient", "cores": threads, "kind": "port", "sample": sample}
+    # near-field corrections of the fast operators on the full mesh (cheap: one
+    # same-triangle quadrature per triangle)
+    pf = oracle.Problem.from_shape(shape, ref, param)
+    t0 = time.perf_counter()
+    pf.near_corrections(complex(ka), ka * m)
+    t_near = time.perf_counter() - t0
+    return {"value": total / n_orient, "unit": "s/orient", "cores": threads, "kind": "port", "sample": sample,
+            "near_corrections_ms_full_mesh": t_near * 1e3}
 
 
 def oracle_fixture_errors(ctx, workload, M32, X32):
@@ -193,6 +200,50 @@ def oracle_fixture_errors(ctx, workload, M32, X32):
                        f"currents of RHS columns {list(int(c) for c in cols)}, Mueller 4x4x{g['M'].shape[2]})"}
 
 
+def next_rows(ctx, k_ext, k_int, nrhs, ir_tol):
+    """The rows beside the dense path, on the same mesh and right-hand sides,
+    after the timed steps (each result is its own measurement, not part of
+    s/orient): near-field corrections of the fast operators
+    (bem_fmm.cu:506-686), GMRES on the FP64 Z preconditioned by the FP32
+    factors of Z (block_gmres.h:52-61 with an LU preconditioner) and by the
+    block-Jacobi / RAS preconditioner (precond.cu:407-895, 1413-1556)."""
+    import bem_amd
+
+    out = {}
+    for rep in range(2):  # the first call also builds the Graglia terms of the mesh
+        t0 = time.perf_counter()
+        rp, ci, vals = ctx.near_corrections(k_ext, k_int)
+        ms = (time.perf_counter() - t0) * 1e3
+    out["near_corrections"] = {"ms_wall_incl_copy_out": ms, "nnz": int(len(ci)),
+                               "entries_per_s": 4 * len(ci) / (ms * 1e-3)}
+    log(f"near corrections: {out['near_corrections']}")
+    del rp, ci, vals
+    info = ctx.factor("fp32")
+    if info:
+        raise RuntimeError(f"FP32 factorisation info {info}")
+    for tol in (1e-5, ir_tol):
+        t0 = time.perf_counter()
+        _, st = ctx.solve_gmres(None, tol=tol, restart=30, max_iter=200, return_X=False, nrhs=nrhs,
+                                allow_noconv=True, precond="lu")
+        out[f"gmres_lu32_tol{tol:g}"] = {"iterations": st["iterations"], "converged": bool(st["converged"]),
+                                         "residual": st["residual"], "ms": (time.perf_counter() - t0) * 1e3}
+        log(f"GMRES, FP32 LU preconditioner, tol {tol:g}: {out[f'gmres_lu32_tol{tol:g}']}")
+    # the reference's automatic block-Jacobi parameters for this N and ka
+    auto = bem_amd.auto_precond_params(ctx.N, k_ext.real)
+    nb, build_ms = ctx.block_jacobi(auto["radius_mult"], auto["max_block_rwg"], auto["overlap_layers"])
+    out["block_jacobi"] = {"params": auto, "blocks": nb, "build_ms": build_ms}
+    log(f"block-Jacobi build: {out['block_jacobi']}")
+    for tol in (1e-5, ir_tol):
+        t0 = time.perf_counter()
+        _, st = ctx.solve_gmres(None, tol=tol, restart=30, max_iter=300, return_X=False, nrhs=nrhs,
+                                allow_noconv=True, precond="block")
+        out[f"gmres_block_jacobi_tol{tol:g}"] = {"iterations": st["iterations"], "converged": bool(st["converged"]),
+                                                 "residual": st["residual"],
+                                                 "ms": (time.perf_counter() - t0) * 1e3}
+        log(f"GMRES, block-Jacobi preconditioner, tol {tol:g}: {out[f'gmres_block_jacobi_tol{tol:g}']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +262,8 @@ def main():
                     help="time the steps without the per-launch HIP events (no per-kernel classes or roofline)")
     ap.add_argument("--skip-external", action="store_true",
                     help="skip timing the library FP64 LU (torch.linalg.lu_factor on rocSOLVER)")
+    ap.add_argument("--skip-next-rows", action="store_true",
+                    help="skip the measurements beside the path (near-field corrections, GMRES, block-Jacobi)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the steps, copy MIB MiB device-to-device with one torch kernel (a known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters of profiling runs)")
@@ -330,6 +383,11 @@ def main():
                     mueller_err_fp32ir_vs_fp64=mueller_err(M32, M64))
         log(f"fp64: {fp64}")
 
+    rows = None
+    if not args.skip_next_rows:
+        rows = next_rows(ctx, k_ext, k_int, nrhs, args.ir_tol)
+        log(f"next rows: {rows}")
+
     # GEMM tile plan picked by the autotuner at this size (timed again here, outside the steps)
     tune = ctx.autotune(nrhs)
 
@@ -398,6 +456,7 @@ def main():
             k: {"launches": int(v[0]), "ms": v[1], "tflops": (v[2] / (v[1] * 1e-3) / 1e12) if v[1] else 0.0}
             for k, v in kstats_all.items() if v[0] > 0},
         "adjoint_gradient": adjoint,
+        "beside_the_path": rows,
         "roofline": roof,
     }
     ctx.close()

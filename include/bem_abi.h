@@ -246,7 +246,12 @@ int bem_solve_gmres(bem_ctx* ctx, const double* B, int nrhs, double* X, double t
  * the RWGs within overlap_layers * 2 mean element extents (RAS), and each
  * extended block's 2B x 2B sub-matrix of the device's FP64 Z is factored on
  * the device.  Needs the mesh and the FP64 Z; build_ms (nullable) receives the
- * build time.  Returns the block count or < 0. */
+ * build time.  Returns the block count or < 0.  Cost: each block is inverted
+ * by one workgroup, ~(2 Be)^3 operations for Be extended RWGs, so the build
+ * time grows with the cube of the block size: the reference's automatic size
+ * (AutoPrecondParams::compute, precond.cu:896-970: 180 at N = 9216) builds
+ * the 80 blocks of a 2N = 18432 mesh in ~2 s, while max_block_rwg = 1500 on
+ * that mesh gives 10 blocks extended to up to 3106 RWGs and takes minutes. */
 int bem_block_jacobi(bem_ctx* ctx, double radius_mult, int max_block_rwg, int overlap_layers, double* build_ms);
 /* The same partition on the host without a context (precond.cu:414-605):
  * returns the block count; *n_ext receives the total extended size; sizes
