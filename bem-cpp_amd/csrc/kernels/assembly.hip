@@ -30,25 +30,66 @@
 //
 // Kernels, in launch order:
 //   k_graglia      P(r_q), V(r_q) at every triangle's own points      (FP64)
-//   k_tripair_far  non-touching triangle pairs -> H records   (FP32 trig, FP64 sums)
-//   k_tripair_near touching pairs incl. self    -> H records   (FP64)
-//   k_gather       Z[m][n] = Z[n][m] = sum of the 4 half-pair records of (m, n)
+//   k_tripair_far  non-touching triangle pairs -> H slots     (FP32 trig, FP64 sums)
+//   k_tripair_near touching pairs incl. self    -> H slots     (FP64)
+//   k_gather       Z[m][n] = Z[n][m] = sum of the 4 half-pair slots of (m, n)
 //                  in a fixed order; PMCHWT blocks formed in place, FP64 and FP32
 //   k_pad          identity on the padding rows/columns
-// H record of pair (t <= s): 9 half pairs (a on t, b on s) x {P, Q, Ks} complex,
+// Half-pair value (a on t, b on s): {P, Q, Ks} complex (6 doubles),
 // P = eta_e L_e + eta_i L_i, Q = L_e/eta_e + L_i/eta_i, Ks = K_e + K_i.
-// No atomics: every record has one writer, every Z entry one reader-summer.
+// A triangle pair's 9 values go to the RWG pairs that read them: H holds, for
+// every RWG pair m <= n, the 4 slots of its half combinations (hm, hn) side
+// by side (192 bytes), grouped by 16 x 16 tiles of (m, n) (slot_index), so
+// the gather and the contraction read each slot once with coalesced loads and
+// the scatter is paid by the VALU-bound producers.
+// No atomics: every slot has one writer, every Z entry one reader-summer.
 #include "../bem_internal.h"
 
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kInv4Pi = 1.0 / (4.0 * kPi);
-constexpr int kRec = 54;  // doubles per H record
+constexpr int kRec = 54;  // doubles of the 9 half-pair values of one triangle pair
+constexpr int kSlot = 6;  // doubles per half-pair slot: P, Q, Ks complex
+constexpr int kGt = 16;   // gather / contraction tile edge (RWGs)
 
-__device__ __forceinline__ long long rec_index(int t, int s, int T) {
-    // packed upper triangle; row t holds s = t..T-1
-    return (long long)t * T - (long long)t * (t - 1) / 2 + (s - t);
+// Slot of RWG pair (m <= n), halves hm of m and hn of n (0: plus, 1: minus):
+// the 4 slots of a pair together, the pairs of a 16 x 16 tile of (m, n) in
+// row-major order, the tiles with m tile <= n tile packed by tile rows.
+__device__ __forceinline__ long long slot_index(int m, int n, int hm, int hn, int ntile) {
+    const long long bi = m / kGt, bj = n / kGt;
+    const long long tile = bi * ntile - bi * (bi - 1) / 2 + (bj - bi);
+    return (tile * (kGt * kGt) + (m % kGt) * kGt + (n % kGt)) * 4 + hm * 2 + hn;
+}
+
+// Slot that reads the value of (a on t, b on s), t <= s, or -1 when none does
+// (a self pair's (a, b) with RWG m > n: its (b, a) value serves the pair).
+// The two halves of one RWG on two triangles (m == n, t != s) give both half
+// orders of (m, m) the same value: *dup receives the second slot.
+__device__ __forceinline__ long long half_pair_slot(const int* __restrict__ hrwg, const double* __restrict__ hc,
+                                                    int t, int a, int s, int b, int ntile, long long* dup) {
+    *dup = -1;
+    const int m = hrwg[t * 3 + a], n = hrwg[s * 3 + b];
+    if (m < 0 || n < 0) return -1;
+    const int hm = hc[t * 3 + a] < 0.0, hn = hc[s * 3 + b] < 0.0;  // c < 0 on the minus half
+    if (m < n) return slot_index(m, n, hm, hn, ntile);
+    if (m > n) return t == s ? -1 : slot_index(n, m, hn, hm, ntile);
+    if (t != s) *dup = slot_index(m, m, hn, hm, ntile);
+    return slot_index(m, m, hm, hn, ntile);
+}
+
+__device__ __forceinline__ void store_slot(double* __restrict__ H, long long slot, long long dup, double2 v0,
+                                           double2 v1, double2 v2) {
+    double2* o = reinterpret_cast<double2*>(H + slot * kSlot);
+    o[0] = v0;
+    o[1] = v1;
+    o[2] = v2;
+    if (dup >= 0) {
+        o = reinterpret_cast<double2*>(H + dup * kSlot);
+        o[0] = v0;
+        o[1] = v1;
+        o[2] = v2;
+    }
 }
 
 // sin/cos on [-pi/4, pi/4]: Taylor series to x^9 / x^10 (truncation < 2e-9).
@@ -192,18 +233,20 @@ __device__ __forceinline__ void half_pair(const Mom& m, double kr, double ki, do
     K.i = cc * km[1];
 }
 
-// Writes the 9 half-pair values of pair (t, s); `corr` (self pairs only)
-// holds the symmetrised static part per wavenumber and half pair.
-__device__ __forceinline__ void write_record(double* __restrict__ H, long long rec, const Mom* mom,
-                                             const AsmParams& p, const double* __restrict__ hc,
+// Writes the 9 half-pair values of pair (t, s) to their slots; `corr` (self
+// pairs only) holds the symmetrised static part per wavenumber and half pair.
+__device__ __forceinline__ void write_record(double* __restrict__ H, int ntile, const int* __restrict__ hrwg,
+                                             const Mom* mom, const AsmParams& p, const double* __restrict__ hc,
                                              const double* __restrict__ hp, int t, int s, const C2* corr) {
-    double* out = H + rec * kRec;
 #pragma unroll 1
     for (int a = 0; a < 3; a++) {
         const double ca = hc[t * 3 + a];
         const double pa[3] = {hp[(t * 3 + a) * 3], hp[(t * 3 + a) * 3 + 1], hp[(t * 3 + a) * 3 + 2]};
 #pragma unroll 1
         for (int b = 0; b < 3; b++) {
+            long long dup;
+            const long long slot = half_pair_slot(hrwg, hc, t, a, s, b, ntile, &dup);
+            if (slot < 0) continue;
             const double cb = hc[s * 3 + b];
             const double pb[3] = {hp[(s * 3 + b) * 3], hp[(s * 3 + b) * 3 + 1], hp[(s * 3 + b) * 3 + 2]};
             C2 L[2], K[2];
@@ -215,10 +258,10 @@ __device__ __forceinline__ void write_record(double* __restrict__ H, long long r
                     L[kk].i += corr[kk * 9 + a * 3 + b].i;
                 }
             }
-            double2* o = reinterpret_cast<double2*>(out + (a * 3 + b) * 6);
-            o[0] = make_double2(p.eta[0] * L[0].r + p.eta[1] * L[1].r, p.eta[0] * L[0].i + p.eta[1] * L[1].i);
-            o[1] = make_double2(L[0].r / p.eta[0] + L[1].r / p.eta[1], L[0].i / p.eta[0] + L[1].i / p.eta[1]);
-            o[2] = make_double2(K[0].r + K[1].r, K[0].i + K[1].i);
+            store_slot(H, slot, dup,
+                       make_double2(p.eta[0] * L[0].r + p.eta[1] * L[1].r, p.eta[0] * L[0].i + p.eta[1] * L[1].i),
+                       make_double2(L[0].r / p.eta[0] + L[1].r / p.eta[1], L[0].i / p.eta[0] + L[1].i / p.eta[1]),
+                       make_double2(K[0].r + K[1].r, K[0].i + K[1].i));
         }
     }
 }
@@ -232,16 +275,18 @@ __device__ __forceinline__ void cmac(double acc[2], const double a[2], double x,
 // Writes the 9 half-pair values of pair (t, s) combined with q (RecCoef);
 // `sv` (self pairs, moments of G only) holds the static parts vec_ab (9
 // values) and S_P (sv[9]), which enter U and V with c_a c_b / (4 pi).
-__device__ __forceinline__ void write_record_coef(double* __restrict__ H, long long rec, const Mom* mom,
-                                                  const RecCoef& q, const double* __restrict__ hc,
+__device__ __forceinline__ void write_record_coef(double* __restrict__ H, int ntile, const int* __restrict__ hrwg,
+                                                  const Mom* mom, const RecCoef& q, const double* __restrict__ hc,
                                                   const double* __restrict__ hp, int t, int s, const double* sv) {
-    double* out = H + rec * kRec;
 #pragma unroll 1
     for (int a = 0; a < 3; a++) {
         const double ca = hc[t * 3 + a];
         const double pa[3] = {hp[(t * 3 + a) * 3], hp[(t * 3 + a) * 3 + 1], hp[(t * 3 + a) * 3 + 2]};
 #pragma unroll 1
         for (int b = 0; b < 3; b++) {
+            long long dup;
+            const long long slot = half_pair_slot(hrwg, hc, t, a, s, b, ntile, &dup);
+            if (slot < 0) continue;
             const double cb = hc[s * 3 + b];
             const double pb[3] = {hp[(s * 3 + b) * 3], hp[(s * 3 + b) * 3 + 1], hp[(s * 3 + b) * 3 + 2]};
             const double cc = ca * cb;
@@ -261,10 +306,7 @@ __device__ __forceinline__ void write_record_coef(double* __restrict__ H, long l
                 cmac(Q, q.bQ[w], V[0], V[1]);
                 cmac(K, q.cK[w], cc * km[0], cc * km[1]);
             }
-            double2* o = reinterpret_cast<double2*>(out + (a * 3 + b) * 6);
-            o[0] = make_double2(P[0], P[1]);
-            o[1] = make_double2(Q[0], Q[1]);
-            o[2] = make_double2(K[0], K[1]);
+            store_slot(H, slot, dup, make_double2(P[0], P[1]), make_double2(Q[0], Q[1]), make_double2(K[0], K[1]));
         }
     }
 }
@@ -283,7 +325,8 @@ __device__ __forceinline__ bool touching(const int4& a, const int4& b) {
 template <int NQ, bool LOSSY, int MODE>
 __global__ __launch_bounds__(256) void k_tripair_far(int T, const double4* __restrict__ qp,
                                                      const int4* __restrict__ tv, const double* __restrict__ hc,
-                                                     const double* __restrict__ hp, AsmParams p, RecCoefSet q,
+                                                     const double* __restrict__ hp, const int* __restrict__ hrwg,
+                                                     int ntile, AsmParams p, RecCoefSet q,
                                                      double* __restrict__ H) {
     const int bt = blockIdx.y, bs = blockIdx.x;
     if (bt > bs) return;
@@ -359,11 +402,10 @@ __global__ __launch_bounds__(256) void k_tripair_far(int T, const double4* __res
         mom_fold(mom[1], in1, P);
     }
     if (MODE == 0) {
-        write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+        write_record(H, ntile, hrwg, mom, p, hc, hp, t, s, nullptr);
     } else {
 #pragma unroll 1
-        for (int j = 0; j < q.n; j++)
-            write_record_coef(q.H[j], rec_index(t, s, T), mom, q.q[j], hc, hp, t, s, nullptr);
+        for (int j = 0; j < q.n; j++) write_record_coef(q.H[j], ntile, hrwg, mom, q.q[j], hc, hp, t, s, nullptr);
     }
 }
 
@@ -467,8 +509,9 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
                                                       const double4* __restrict__ qp,
                                                       const double* __restrict__ hc,
                                                       const double* __restrict__ hp,
-                                                      const double4* __restrict__ pv, AsmParams p, RecCoefSet q,
-                                                      double* __restrict__ H) {
+                                                      const double4* __restrict__ pv,
+                                                      const int* __restrict__ hrwg, int ntile, AsmParams p,
+                                                      RecCoefSet q, double* __restrict__ H) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_near) return;
     const int t = pairs[idx].x, s = pairs[idx].y;
@@ -537,12 +580,12 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
         }
         if (!self || MODE == 2) {
             if (MODE == 0) {
-                write_record(H, rec_index(t, s, T), mom, p, hc, hp, t, s, nullptr);
+                write_record(H, ntile, hrwg, mom, p, hc, hp, t, s, nullptr);
             } else {
                 const int j0 = nmom > 1 ? js : 0, j1 = nmom > 1 ? js + 1 : q.n;
 #pragma unroll 1
                 for (int j = j0; j < j1; j++)
-                    write_record_coef(q.H[j], rec_index(t, s, T), mom, q.q[j], hc, hp, t, s, nullptr);
+                    write_record_coef(q.H[j], ntile, hrwg, mom, q.q[j], hc, hp, t, s, nullptr);
             }
             continue;
         }
@@ -585,30 +628,27 @@ __global__ __launch_bounds__(128) void k_tripair_near(int T, int n_near, const i
                 }
             }
         if (MODE == 0) {
-            write_record(H, rec_index(t, t, T), mom, p, hc, hp, t, t, corr);
+            write_record(H, ntile, hrwg, mom, p, hc, hp, t, t, corr);
         } else {
 #pragma unroll 1
-            for (int j = 0; j < q.n; j++)
-                write_record_coef(q.H[j], rec_index(t, t, T), mom, q.q[j], hc, hp, t, t, sv);
+            for (int j = 0; j < q.n; j++) write_record_coef(q.H[j], ntile, hrwg, mom, q.q[j], hc, hp, t, t, sv);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Z[m][n] from the four half-pair records of RWGs m and n, PMCHWT blocks:
+// Z[m][n] from the four half-pair slots of RWGs m and n, PMCHWT blocks:
 // Z[m][n] = P, Z[m][N+n] = -Ks, Z[N+m][n] = Ks, Z[N+m][N+n] = Q.
-// The records are shared by (m, n) and (n, m) (P, Q and Ks are symmetric in
+// The slots are shared by (m, n) and (n, m) (P, Q and Ks are symmetric in
 // the RWG pair), so one read of them gives both: block = 16 x 16 tile of
 // (m, n) with the m tile at or before the n tile, thread (i, j) sums the pair
-// (tm0 + i, tn0 + j) when m <= n and writes its row segment; the (n, m)
-// entries go out as row segments of the mirrored tile through LDS.
+// (tm0 + i, tn0 + j) when m <= n (its 4 slots: 192 contiguous bytes, the
+// block's 48 KB contiguous) and writes its row segment; the (n, m) entries go
+// out as row segments of the mirrored tile through LDS.
 // ---------------------------------------------------------------------------
-constexpr int kGt = 16;  // gather tile edge (RWGs)
-
-__global__ __launch_bounds__(256) void k_gather(int N, int T, int ld, const int4* __restrict__ rwg_half,
-                                                const double* __restrict__ H, double* __restrict__ Zr,
-                                                double* __restrict__ Zi, float* __restrict__ Wr,
-                                                float* __restrict__ Wi) {
+__global__ __launch_bounds__(256) void k_gather(int N, int ld, const double* __restrict__ H,
+                                                double* __restrict__ Zr, double* __restrict__ Zi,
+                                                float* __restrict__ Wr, float* __restrict__ Wi) {
     if (blockIdx.y > blockIdx.x) return;
     __shared__ double sv[6][kGt][kGt + 1];  // (P, Q, Ks) x (re, im) of pair (i, j)
     const int tm0 = blockIdx.y * kGt, tn0 = blockIdx.x * kGt;
@@ -617,29 +657,17 @@ __global__ __launch_bounds__(256) void k_gather(int N, int T, int ld, const int4
     const bool own = m < N && n < N && m <= n;
     double P0 = 0, P1 = 0, Q0 = 0, Q1 = 0, K0 = 0, K1 = 0;
     if (own) {
-        const int4 hm = rwg_half[m], hn = rwg_half[n];
-        const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
-        const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
+        // slots in the order (hm, hn) = (+, +), (+, -), (-, +), (-, -)
+        const double2* h = reinterpret_cast<const double2*>(H + slot_index(m, n, 0, 0, gridDim.x) * kSlot);
+        double2 v[12];
 #pragma unroll
-        for (int sa = 0; sa < 2; sa++)
+        for (int k = 0; k < 12; k++) v[k] = h[k];
 #pragma unroll
-            for (int sb = 0; sb < 2; sb++) {
-                const int ta = tm[sa], tb = tn[sb];
-                long long rec;
-                int slot;
-                if (ta <= tb) {
-                    rec = rec_index(ta, tb, T);
-                    slot = lm[sa] * 3 + ln[sb];
-                } else {
-                    rec = rec_index(tb, ta, T);
-                    slot = ln[sb] * 3 + lm[sa];
-                }
-                const double2* h = reinterpret_cast<const double2*>(H + rec * kRec + slot * 6);
-                const double2 a = h[0], b = h[1], c = h[2];
-                P0 += a.x; P1 += a.y;
-                Q0 += b.x; Q1 += b.y;
-                K0 += c.x; K1 += c.y;
-            }
+        for (int sl = 0; sl < 4; sl++) {
+            P0 += v[3 * sl].x; P1 += v[3 * sl].y;
+            Q0 += v[3 * sl + 1].x; Q1 += v[3 * sl + 1].y;
+            K0 += v[3 * sl + 2].x; K1 += v[3 * sl + 2].y;
+        }
         const size_t r0 = (size_t)m * ld, r1 = (size_t)(N + m) * ld;
         if (Zr) {
             Zr[r0 + n] = P0;      Zi[r0 + n] = P1;
@@ -706,12 +734,12 @@ struct RecArrays {
     const double* p[kMaxRecSets];
 };
 
-constexpr int kCt = 16;  // contraction tile edge (RWGs)
+constexpr int kCt = kGt;  // contraction tile edge (RWGs): the slot tiles
 
 template <int NS>
-__global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __restrict__ rwg_half, RecArrays H,
-                                                  const double* __restrict__ Wr, const double* __restrict__ Wi,
-                                                  long ldw, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_contract(int N, RecArrays H, const double* __restrict__ Wr,
+                                                  const double* __restrict__ Wi, long ldw,
+                                                  double* __restrict__ part) {
     __shared__ double wt[8][kCt][kCt + 1];  // W_b[n][m], b = 0..3, (re, im)
     __shared__ double red[2 * NS][256];
     const int tm0 = blockIdx.y * kCt, tn0 = blockIdx.x * kCt;
@@ -740,35 +768,21 @@ __global__ __launch_bounds__(256) void k_contract(int N, int T, const int4* __re
 #pragma unroll
     for (int q = 0; q < NS; q++) acc[q][0] = acc[q][1] = 0.0;
     if (m < N && n < N && m <= n) {
-        const int4 hm = rwg_half[m], hn = rwg_half[n];
-        const int tm[2] = {hm.x, hm.z}, lm[2] = {hm.y, hm.w};
-        const int tn[2] = {hn.x, hn.z}, ln[2] = {hn.y, hn.w};
         double P[NS][2], Q[NS][2], K[NS][2];
 #pragma unroll
         for (int q = 0; q < NS; q++) P[q][0] = P[q][1] = Q[q][0] = Q[q][1] = K[q][0] = K[q][1] = 0.0;
+        const long long s0 = slot_index(m, n, 0, 0, gridDim.x) * kSlot;
 #pragma unroll
-        for (int sa = 0; sa < 2; sa++)
+        for (int q = 0; q < NS; q++) {
+            const double2* h = reinterpret_cast<const double2*>(H.p[q] + s0);
 #pragma unroll
-            for (int sb = 0; sb < 2; sb++) {
-                const int ta = tm[sa], tb = tn[sb];
-                long long rec;
-                int slot;
-                if (ta <= tb) {
-                    rec = rec_index(ta, tb, T);
-                    slot = lm[sa] * 3 + ln[sb];
-                } else {
-                    rec = rec_index(tb, ta, T);
-                    slot = ln[sb] * 3 + lm[sa];
-                }
-#pragma unroll
-                for (int q = 0; q < NS; q++) {
-                    const double2* h = reinterpret_cast<const double2*>(H.p[q] + rec * kRec + slot * 6);
-                    const double2 a = h[0], b = h[1], c = h[2];
-                    P[q][0] += a.x; P[q][1] += a.y;
-                    Q[q][0] += b.x; Q[q][1] += b.y;
-                    K[q][0] += c.x; K[q][1] += c.y;
-                }
+            for (int sl = 0; sl < 4; sl++) {  // (hm, hn) = (+, +), (+, -), (-, +), (-, -)
+                const double2 a = h[3 * sl], b = h[3 * sl + 1], c = h[3 * sl + 2];
+                P[q][0] += a.x; P[q][1] += a.y;
+                Q[q][0] += b.x; Q[q][1] += b.y;
+                K[q][0] += c.x; K[q][1] += c.y;
             }
+        }
         const size_t r0 = (size_t)m * ldw, r1 = (size_t)(N + m) * ldw;
         const double f = m < n ? 1.0 : 0.0;  // the diagonal counts once
         const double w00r = Wr[r0 + n] + f * wt[0][j][i], w00i = Wi[r0 + n] + f * wt[1][j][i];
@@ -826,19 +840,20 @@ void launch_records(bem_ctx* c, const AsmParams& p, const RecCoefSet& q, bool lo
         hipLaunchKernelGGL(k_graglia<NQ>, dim3((T * NQ + 255) / 256), dim3(256), 0, c->stream, T, d.qp, d.tv,
                            dverts, dqw, d.pv);
     const int nb = (T + 15) / 16;
+    const int ntile = (d.N + kGt - 1) / kGt;
     const double npair = (double)T * (T + 1) / 2;
     int kt = kt_mark(c, KS_TRIPAIR_FAR);
     if (lossy)
         hipLaunchKernelGGL((k_tripair_far<NQ, true, MODE>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv,
-                           d.hc, d.hp, p, q, c->H);
+                           d.hc, d.hp, d.hrwg, ntile, p, q, c->H);
     else
         hipLaunchKernelGGL((k_tripair_far<NQ, false, MODE>), dim3(nb, nb), dim3(256), 0, c->stream, T, d.qp, d.tv,
-                           d.hc, d.hp, p, q, c->H);
+                           d.hc, d.hp, d.hrwg, ntile, p, q, c->H);
     kt_close(c, kt, npair * NQ * NQ, npair * kRec * 8.0);  // unit: triangle point pairs
     kt = kt_mark(c, KS_TRIPAIR_NEAR);
     if (d.n_near > 0)
         hipLaunchKernelGGL((k_tripair_near<NQ, MODE>), dim3((d.n_near + 127) / 128), dim3(128), 0, c->stream, T,
-                           d.n_near, d.near, d.qp, d.hc, d.hp, d.pv, p, q, c->H);
+                           d.n_near, d.near, d.qp, d.hc, d.hp, d.pv, d.hrwg, ntile, p, q, c->H);
     kt_close(c, kt, (double)d.n_near * NQ * NQ, (double)d.n_near * kRec * 8.0);
 }
 
@@ -846,14 +861,12 @@ template <int NQ>
 int launch_nq(bem_ctx* c, const AsmParams& p, bool lossy, double* Zr, double* Zi, float* Wr, float* Wi,
               const double* dverts, const double* dqw) {
     const DevMesh& d = c->dm;
-    const int T = d.T;
     launch_records<NQ, 0>(c, p, RecCoefSet{}, lossy, dverts, dqw);
     const int N = d.N;
     const int kt = kt_mark(c, KS_GATHER);
     const int gt = (N + kGt - 1) / kGt;
-    hipLaunchKernelGGL(k_gather, dim3(gt, gt), dim3(256), 0, c->stream, N, T, c->npad, d.rwg_half, c->H,
-                       Zr, Zi, Wr, Wi);
-    // records of each unordered RWG pair read once, every Z entry written once
+    hipLaunchKernelGGL(k_gather, dim3(gt, gt), dim3(256), 0, c->stream, N, c->npad, c->H, Zr, Zi, Wr, Wi);
+    // the slots of each unordered RWG pair read once, every Z entry written once
     kt_close(c, kt, 0.0, (double)N * N * (2.0 * 48.0 + (Zr ? 64.0 : 0.0) + (Wr ? 32.0 : 0.0)));
     const int n = 2 * N;
     if (c->npad > n) {
@@ -889,11 +902,16 @@ static int asm_prepare_graglia(bem_ctx* c, const double** dverts, const double**
     return 0;
 }
 
-// Record buffer of T(T+1)/2 records, and the k_graglia inputs.
+// Bytes of one slot array: 4 slots per RWG pair of every tile with m tile <=
+// n tile (slot_index).
+static size_t slot_array_bytes(int N) {
+    const size_t nt = (size_t)((N + kGt - 1) / kGt);
+    return nt * (nt + 1) / 2 * kGt * kGt * 4 * kSlot * sizeof(double);
+}
+
+// Slot buffer and the k_graglia inputs.
 static int asm_prepare(bem_ctx* c, const double** dverts, const double** dqw) {
-    const DevMesh& d = c->dm;
-    const size_t T = (size_t)d.T;
-    const size_t need = T * (T + 1) / 2 * kRec * sizeof(double);
+    const size_t need = slot_array_bytes(c->dm.N);
     if (c->H_bytes < need) {
         if (c->H) (void)hipFree(c->H);
         c->H = nullptr;
@@ -1160,10 +1178,7 @@ int z_entries_launch(bem_ctx* c, const int* rows, const int* cols, int count, do
     return 0;
 }
 
-size_t asm_record_bytes(const bem_ctx* c) {
-    const size_t T = (size_t)c->dm.T;
-    return T * (T + 1) / 2 * kRec * sizeof(double);
-}
+size_t asm_record_bytes(const bem_ctx* c) { return slot_array_bytes(c->dm.N); }
 
 int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq, double* extra, const double* Wr,
                         const double* Wi, long ldw, double* out) {
@@ -1201,14 +1216,12 @@ int asm_contract_launch(bem_ctx* c, const AsmParams& p, const RecCoef* q, int nq
     if (hipMalloc(&part, (np + 1) * ncomp * sizeof(double)) != hipSuccess)
         return bem_fail(c, BEM_ERR_NOMEM, "contraction partial sums");
     const dim3 grid(gx, gx);
-    const int4* rh = c->dm.rwg_half;
-    const int T = c->dm.T;
     if (nq == 1)
-        hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
+        hipLaunchKernelGGL(k_contract<1>, grid, dim3(256), 0, c->stream, N, ra, Wr, Wi, ldw, part);
     else if (nq == 2)
-        hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
+        hipLaunchKernelGGL(k_contract<2>, grid, dim3(256), 0, c->stream, N, ra, Wr, Wi, ldw, part);
     else
-        hipLaunchKernelGGL(k_contract<3>, grid, dim3(256), 0, c->stream, N, T, rh, ra, Wr, Wi, ldw, part);
+        hipLaunchKernelGGL(k_contract<3>, grid, dim3(256), 0, c->stream, N, ra, Wr, Wi, ldw, part);
     hipLaunchKernelGGL(k_sum_parts, dim3(ncomp), dim3(1024), 0, c->stream, np, ncomp, part, part + np * ncomp);
     const hipError_t e1 =
         hipMemcpyAsync(out, part + np * ncomp, ncomp * sizeof(double), hipMemcpyDeviceToHost, c->stream);
