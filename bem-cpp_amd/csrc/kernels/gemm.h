@@ -611,7 +611,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_cgemm_short(int M, int N, const T* __restrict__ Ar, const T* __restrict__ Ai,
                                                      long lda, const T* __restrict__ Br, const T* __restrict__ Bi,
                                                      long ldb, T* __restrict__ Cr, T* __restrict__ Ci, long ldc,
-                                                     int mode) {
+                                                     int mode, T* __restrict__ Ctr, T* __restrict__ Cti, long ldt) {
     constexpr bool M3 = sizeof(T) == 4;
     constexpr int NACC = M3 ? 3 : 2;
     constexpr int NS = kShortKW / 16;  // K steps per wave
@@ -687,6 +687,14 @@ __global__ __launch_bounds__(256) void k_cgemm_short(int M, int N, const T* __re
         Cr[o] += vr;
         Ci[o] += vi;
     }
+    if (Ctr) {  // mode 1 only: C^T as well, consecutive threads on consecutive rows of C
+        const int tr = threadIdx.x % 16, tc = threadIdx.x / 16;
+        const T ur = ((red[0][0][tr][tc] + red[1][0][tr][tc]) + red[2][0][tr][tc]) + red[3][0][tr][tc];
+        const T ui = ((red[0][1][tr][tc] + red[1][1][tr][tc]) + red[2][1][tr][tc]) + red[3][1][tr][tc];
+        const long ot = (long)(n0 + tc) * ldt + m0 + tr;
+        Ctr[ot] = ur;
+        Cti[ot] = ui;
+    }
 }
 
 // Whether k_cgemm_short takes the product: K = 4 kShortKW, M and N multiples
@@ -698,10 +706,13 @@ inline bool cgemm_short_fits(int M, int N, int K, const T* Ar, const T* Ai, long
 }
 
 // Short-K product on the context stream with per-launch timing; products
-// the kernel does not take go to cgemm_ctx.
+// the kernel does not take go to cgemm_ctx.  With Ctr (mode 1, and only for
+// products the kernel takes) C^T is written to (Ctr, Cti) too, N x M with
+// leading dimension ldt.
 template <typename T>
 inline void cgemm_short_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai, long lda, const T* Br,
-                            const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode) {
+                            const T* Bi, long ldb, T* Cr, T* Ci, long ldc, int mode, T* Ctr = nullptr,
+                            T* Cti = nullptr, long ldt = 0) {
     if (!cgemm_short_fits<T>(M, N, K, Ar, Ai, lda)) {
         cgemm_ctx<T>(c, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
         return;
@@ -710,7 +721,7 @@ inline void cgemm_short_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const 
     const long tiles = (long)(M / 16) * (N / 16);
     const unsigned grid = (unsigned)(((tiles + 7) / 8) * 8);
     hipLaunchKernelGGL(k_cgemm_short<T>, dim3(grid), dim3(256), 0, c->stream, M, N, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci,
-                       ldc, mode);
+                       ldc, mode, mode == 1 ? Ctr : nullptr, Cti, ldt);
     const double mnk = (double)M * N * K;
     const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
     kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);

@@ -220,11 +220,14 @@ typedef NtCfg<64, 64, 32, 32, 16, 3> NtCfg0;
 typedef NtCfg<64, 64, 32, 32, 16, 2> NtCfg1;
 typedef NtCfg<128, 64, 64, 32, 16, 1> NtCfg2;
 typedef NtCfg<64, 64, 32, 32, 32, 2> NtCfg3;
+// Not an update configuration: the narrow products of the triangular solves
+// (N = right-hand sides), where 32 x 32 tiles give four times the blocks.
+typedef NtCfg<32, 32, 16, 16, 32, 4> NtCfg4;
 
 // Whether configuration cfg takes the product (shape and alignment).
 inline bool cupdate_nt_fits(int cfg, int M, int N, int K, const float* Ar, const float* Ai, long lda,
                             const float* Btr, const float* Bti, long ldb, const float* Cr, const float* Ci, long ldc) {
-    const int bm = cfg == 2 ? 128 : 64, bn = 64, bk = cfg == 3 ? 32 : 16;
+    const int bm = cfg == 2 ? 128 : cfg == 4 ? 32 : 64, bn = cfg == 4 ? 32 : 64, bk = cfg >= 3 ? 32 : 16;
     auto al = [](const float* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
     return M > 0 && N > 0 && M % bm == 0 && N % bn == 0 && K % bk == 0 && lda % 4 == 0 && ldb % 4 == 0 &&
            ldc % 4 == 0 && al(Ar) && al(Ai) && al(Btr) && al(Bti) && al(Cr) && al(Ci);
@@ -248,6 +251,7 @@ inline void cupdate_nt(int cfg, hipStream_t s, int M, int N, int K, const float*
         case 1: cupdate_nt_cfg<NtCfg1>(s, M, N, K, Ar, Ai, lda, Btr, Bti, ldb, Cr, Ci, ldc, cap); break;
         case 2: cupdate_nt_cfg<NtCfg2>(s, M, N, K, Ar, Ai, lda, Btr, Bti, ldb, Cr, Ci, ldc, cap); break;
         case 3: cupdate_nt_cfg<NtCfg3>(s, M, N, K, Ar, Ai, lda, Btr, Bti, ldb, Cr, Ci, ldc, cap); break;
+        case 4: cupdate_nt_cfg<NtCfg4>(s, M, N, K, Ar, Ai, lda, Btr, Bti, ldb, Cr, Ci, ldc, cap); break;
         default: cupdate_nt_cfg<NtCfg0>(s, M, N, K, Ar, Ai, lda, Btr, Bti, ldb, Cr, Ci, ldc, cap); break;
     }
 }
