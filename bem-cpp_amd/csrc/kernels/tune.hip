@@ -95,7 +95,7 @@ std::map<std::tuple<int, int, int, int>, GemmPlan> g_cache;
 // factorisations, the trials too.  kPlanFormat changes whenever the tile
 // configurations change, so plans of another build are ignored.
 // ---------------------------------------------------------------------------
-constexpr int kPlanFormat = 4;
+constexpr int kPlanFormat = 5;
 
 std::string plan_file() {
     if (const char* e = std::getenv("BEM_TUNE_CACHE")) return e;

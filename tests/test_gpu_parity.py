@@ -313,7 +313,7 @@ def test_every_gemm_tile_and_capped_grid_factor_exactly(ctx, cfg):
 def test_factor_trials_pick_a_tile_and_keep_results(ctx):
     # from 8192 unknowns on, the first factorisation of each precision times
     # one whole trial factorisation per tile configuration (on the context's
-    # own matrix) and keeps the fastest (ties within 3 % go to the tile that
+    # own matrix) and keeps the fastest (ties within 1 % go to the tile that
     # is faster alone); the factors are the same either way
     rs = np.random.RandomState(12)
     n = 8192
@@ -326,9 +326,9 @@ def test_factor_trials_pick_a_tile_and_keep_results(ctx):
         assert info["trialed"], info
         tiles = ctx.UPDATE_TILES32 if prec == "fp32" else ctx.GEMM_TILES64[:4]  # the 32 x 32 tile is not tried
         assert sorted(info["trial_factor_ms"]) == sorted(tiles), info
-        # the fastest trial, or one within 3 % of it
+        # the fastest trial, or one within 1 % of it
         ms = info["trial_factor_ms"]
-        assert ms[info["update"]] <= 1.03 * min(ms.values()) + 1e-9, info
+        assert ms[info["update"]] <= 1.01 * min(ms.values()) + 1e-9, info
         X, st = ctx.solve(B, ir_tol=1e-12, ir_max=30 if prec == "fp32" else 0)
         be = np.abs(Z @ X - B).max() / (np.abs(Z).max() * np.abs(X).max() * n)
         # partial pivoting's growth on a random matrix rises with n: LAPACK
