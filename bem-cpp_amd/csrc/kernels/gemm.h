@@ -421,6 +421,115 @@ inline int gemm_choose(const bem_ctx* c, int M, int N) {
     return gemm_blocks<T>(p.skinny, M, N) >= p.cus ? p.skinny : kGemmSmall;
 }
 
+// ---------------------------------------------------------------------------
+// FP32 products with a short K (K = 16 KQ, at most 128): C (M x N) op= A B in
+// the 3M form, A k-contiguous (row-major, 16-byte aligned rows), B row-major.
+// No LDS and no barrier: every wave owns one 16 x 16 tile of C over the whole
+// K and issues all of its operand loads before the first MFMA (A as 16-byte
+// pieces of its row, B as single values of its column).  For the products on
+// the factorisation's panel chain (the rank-32 updates of a panel step, the
+// L11^{-1} doubling), which run beside the trailing update: there a kernel
+// that stages operands through LDS with a barrier per slice slows down about
+// three times.  The MFMA takes the B fragment first, so each lane ends with
+// four consecutive columns of one row of C.  Batched over blockIdx.y with
+// strides sa, sb, sc.  mode & 3: 0 C -= AB, 1 C = AB, 2 C += AB, 3 C = -AB;
+// mode & GEMM_MODE_URGENT raises the waves' issue priority.
+// ---------------------------------------------------------------------------
+template <int KQ>
+__global__ __launch_bounds__(256) void k_cgemm_rd(int M, int N, const float* __restrict__ Ar,
+                                                  const float* __restrict__ Ai, long lda,
+                                                  const float* __restrict__ Br, const float* __restrict__ Bi,
+                                                  long ldb, float* __restrict__ Cr, float* __restrict__ Ci, long ldc,
+                                                  int mode, long sa, long sb, long sc) {
+    if (mode & GEMM_MODE_URGENT) __builtin_amdgcn_s_setprio(3);
+    const int ntc = N / 16, ntiles = (M / 16) * ntc;
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;
+    const long z = blockIdx.y;
+    Ar += z * sa;
+    Ai += z * sa;
+    Br += z * sb;
+    Bi += z * sb;
+    Cr += z * sc;
+    Ci += z * sc;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int m0 = (tile / ntc) * 16, n0 = (tile % ntc) * 16;
+    const int op = mode & 3;
+    v4f a0[KQ], a1[KQ];
+    float b0[KQ][4], b1[KQ][4];
+    const long oa = (long)(m0 + r) * lda + 4 * g;
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+        a0[q] = *reinterpret_cast<const v4f*>(Ar + oa + 16 * q);
+        a1[q] = *reinterpret_cast<const v4f*>(Ai + oa + 16 * q);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const long ob = (long)(16 * q + 4 * g + e) * ldb + n0 + r;
+            b0[q][e] = Br[ob];
+            b1[q][e] = Bi[ob];
+        }
+    }
+    const long oc = (long)(m0 + r) * ldc + n0 + 4 * g;
+    v4f cr = v4f{0.f, 0.f, 0.f, 0.f}, ci = cr;
+    if (op == 0 || op == 2) {
+        cr = *reinterpret_cast<const v4f*>(Cr + oc);
+        ci = *reinterpret_cast<const v4f*>(Ci + oc);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every load in flight before the first MFMA
+    v4f t1 = v4f{0.f, 0.f, 0.f, 0.f}, t2 = t1, t3 = t1;
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+        const v4f a2 = a0[q] + a1[q];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[q][e], a0[q][e], t1, 0, 0, 0);
+            t2 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1[q][e], a1[q][e], t2, 0, 0, 0);
+            t3 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[q][e] + b1[q][e], a2[e], t3, 0, 0, 0);
+        }
+    }
+    const v4f pr = t1 - t2, pi = t3 - t1 - t2;
+    v4f outr, outi;
+    if (op == 0) {
+        outr = cr - pr;
+        outi = ci - pi;
+    } else if (op == 1) {
+        outr = pr;
+        outi = pi;
+    } else if (op == 2) {
+        outr = cr + pr;
+        outi = ci + pi;
+    } else {
+        outr = -pr;
+        outi = -pi;
+    }
+    *reinterpret_cast<v4f*>(Cr + oc) = outr;
+    *reinterpret_cast<v4f*>(Ci + oc) = outi;
+}
+
+// Whether k_cgemm_rd takes the product (FP32, K in {32, 64, 128}).
+inline bool cgemm_rd_fits(int M, int N, int K, const float* Ar, const float* Ai, long lda, const float* Cr,
+                          const float* Ci, long ldc, long sa = 0, long sc = 0) {
+    auto al = [](const float* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+    return M > 0 && N > 0 && M % 16 == 0 && N % 16 == 0 && (K == 32 || K == 64 || K == 128) && lda % 4 == 0 &&
+           ldc % 4 == 0 && sa % 4 == 0 && sc % 4 == 0 && al(Ar) && al(Ai) && al(Cr) && al(Ci);
+}
+
+inline void cgemm_rd(hipStream_t s, int M, int N, int K, const float* Ar, const float* Ai, long lda, const float* Br,
+                     const float* Bi, long ldb, float* Cr, float* Ci, long ldc, int mode, int nb = 1, long sa = 0,
+                     long sb = 0, long sc = 0) {
+    const long tiles = (long)(M / 16) * (N / 16);
+    const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)nb);
+    if (K == 32)
+        hipLaunchKernelGGL(k_cgemm_rd<2>, grid, dim3(256), 0, s, M, N, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, sa,
+                           sb, sc);
+    else if (K == 64)
+        hipLaunchKernelGGL(k_cgemm_rd<4>, grid, dim3(256), 0, s, M, N, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, sa,
+                           sb, sc);
+    else
+        hipLaunchKernelGGL(k_cgemm_rd<8>, grid, dim3(256), 0, s, M, N, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, sa,
+                           sb, sc);
+}
+
 // Launch flags of cgemm_ctx: GEMM_HEADROOM caps the grid as
 // gemm_headroom_cap does (an update running beside the panel stream);
 // GEMM_PANEL marks a product of the panel stream's critical path: the plan's
@@ -443,7 +552,14 @@ inline void cgemm_ctx(bem_ctx* c, int M, int N, int K, const T* Ar, const T* Ai,
     const int cfg = (flags & GEMM_PANEL) ? plan.panel : gemm_choose<T>(c, M, N);
     const long cap = (flags & GEMM_HEADROOM) ? gemm_headroom_cap(plan, cfg) : 0;
     if (flags & GEMM_PANEL) mode |= GEMM_MODE_URGENT;
-    cgemm_any<T>(cfg, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, cap, upd ? 1 : 0);
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+        if ((flags & GEMM_PANEL) && cgemm_rd_fits(M, N, K, Ar, Ai, lda, Cr, Ci, ldc)) {
+            cgemm_rd(s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode);
+            done = true;
+        }
+    }
+    if (!done) cgemm_any<T>(cfg, s, M, N, K, Ar, Ai, lda, Br, Bi, ldb, Cr, Ci, ldc, mode, 1, 0, 0, 0, cap, upd ? 1 : 0);
     const double mnk = (double)M * N * K;
     const double bytes = 2.0 * sizeof(T) * ((double)M * K + (double)K * N + (mode == 1 ? 1.0 : 2.0) * M * N);
     kt_close(c, k, (sizeof(T) == 4 ? 6.0 : 8.0) * mnk, bytes);

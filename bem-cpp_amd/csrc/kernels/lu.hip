@@ -605,6 +605,16 @@ void invert_lower_doubling(hipStream_t s, const T* Fr, const T* Fi, long ldf, T*
         const long sf = 2L * h * (ldf + 1), si = 2L * h * (NB + 1), st = (long)h * h;
         const long f21 = (long)h * ldf, i21 = (long)h * NB, i22 = (long)h * (NB + 1);
         // T = L21 X11, then X21 = -X22 T
+        if constexpr (sizeof(T) == 4) {  // register-direct short-K products (no LDS, no barrier)
+            if (cgemm_rd_fits(h, h, h, Fr + f21, Fi + f21, ldf, w.tr, w.ti, h, sf, st) &&
+                cgemm_rd_fits(h, h, h, Ir + i22, Ii + i22, NB, Ir + i21, Ii + i21, NB, si, si)) {
+                cgemm_rd(s, h, h, h, Fr + f21, Fi + f21, ldf, Ir, Ii, NB, w.tr, w.ti, h, 1 | GEMM_MODE_URGENT, pairs,
+                         sf, si, st);
+                cgemm_rd(s, h, h, h, Ir + i22, Ii + i22, NB, w.tr, w.ti, h, Ir + i21, Ii + i21, NB,
+                         3 | GEMM_MODE_URGENT, pairs, si, st, si);
+                continue;
+            }
+        }
         cgemm_any<T>(kGemmSmall, s, h, h, h, Fr + f21, Fi + f21, ldf, Ir, Ii, NB, w.tr, w.ti, h, 1 | GEMM_MODE_URGENT,
                      pairs, sf, si, st);
         cgemm_any<T>(kGemmSmall, s, h, h, h, Ir + i22, Ii + i22, NB, w.tr, w.ti, h, Ir + i21, Ii + i21, NB,
