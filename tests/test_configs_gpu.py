@@ -17,9 +17,9 @@ refinement), far field, orientation-averaged Mueller matrix.
 Tolerances:
   * Z entry-wise: |dZ| <= 2e-6 |Z| + 1e-7 max|Z| (far pairs evaluate sin, cos
     and exp in FP32 of an FP64-reduced phase, ~1e-7 relative per point pair);
-  * surface currents: ||x - x_ref|| / ||x_ref|| <= 1e-5 per stored column and
+  * surface currents: ||x - x_ref|| / ||x_ref|| <= 1e-6 per stored column and
     for the norms of all columns (the Z error above, amplified by the
-    conditioning of the system);
+    conditioning of the system; measured 4-5e-8);
   * Mueller: max_ij,theta |dM_ij(theta)| / M11(theta) <= 1e-4 (the gate).
 """
 import numpy as np
@@ -31,7 +31,7 @@ pytestmark = pytest.mark.gpu
 
 MESH = {"c2": ("sphere", 3, 1.0), "c3": ("hex", 3, 1.4286), "c5": ("hex", 4, 0.7)}
 Z_RTOL, Z_ATOL = 2e-6, 1e-7
-X_TOL = 1e-5
+X_TOL = 1e-6
 MUELLER_GATE = 1e-4
 NTHETA = 181
 
