@@ -136,7 +136,7 @@ struct bem_ctx {
     hipStream_t stream2 = nullptr;  // panel stream of the look-ahead factorisation
     hipStream_t stream3 = nullptr;  // updates inside an outer panel off its critical path
     // factorisation stream layout (see factor() in kernels/lu.hip): 0 one
-    // stream, 1 panel + update streams on all CUs, 2 CU-partitioned
+    // stream, 1 panel + update streams on all CUs
     int lu_streams = -1;            // -1: not chosen yet
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t lu_ev[6] = {};  // cross-stream ordering inside the factorisation
