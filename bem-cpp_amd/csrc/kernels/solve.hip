@@ -47,23 +47,45 @@ __global__ __launch_bounds__(256) void k_accumulate(long count, const T* __restr
     }
 }
 
-// out[j] = sum_{i < n} |A[i][j]|^2, one block per column, fixed reduction order.
-__global__ __launch_bounds__(256) void k_colnorm2(int n, const double* __restrict__ Ar, const double* __restrict__ Ai,
-                                                  long ld, double* __restrict__ out) {
-    __shared__ double red[256];
-    const int j = blockIdx.x;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        const double a = Ar[(long)i * ld + j], b = Ai[(long)i * ld + j];
-        s += a * a + b * b;
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+// out[j] = sum_{i < n} |A[i][j]|^2 for j < ncol in two passes with a fixed
+// summation order: block b sums rows [256 b, 256 b + 256) of every column
+// (64 consecutive columns per wave and row, so the loads are coalesced; the
+// four row lanes of a column summed in lane order) into part[b][j]; then
+// k_colnorm2_sum adds the blocks' parts in block order.
+constexpr int kNormRows = 256;
+__global__ __launch_bounds__(256) void k_colnorm2_part(int n, int ncol, const double* __restrict__ Ar,
+                                                       const double* __restrict__ Ai, long ld,
+                                                       double* __restrict__ part) {
+    __shared__ double red[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int r0 = blockIdx.x * kNormRows, r1 = min(n, r0 + kNormRows);
+    for (int c0 = 0; c0 < ncol; c0 += 64) {
+        const int j = c0 + tx;
+        double s = 0.0;
+        if (j < ncol)
+            for (int i = r0 + ty; i < r1; i += 4) {
+                const double a = Ar[(long)i * ld + j], b = Ai[(long)i * ld + j];
+                s += a * a + b * b;
+            }
+        red[ty][tx] = s;
+        __syncthreads();
+        if (ty == 0 && j < ncol) part[(long)blockIdx.x * ncol + j] = ((red[0][tx] + red[1][tx]) + red[2][tx]) + red[3][tx];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[j] = red[0];
+}
+__global__ __launch_bounds__(256) void k_colnorm2_sum(int nblk, int ncol, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= ncol) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(long)b * ncol + j];
+    out[j] = s;
+}
+int colnorm_blocks(int n) { return (n + kNormRows - 1) / kNormRows; }
+void colnorm2(hipStream_t s, int n, int ncol, const double* Ar, const double* Ai, long ld, double* part, double* out) {
+    const int nb = colnorm_blocks(n);
+    hipLaunchKernelGGL(k_colnorm2_part, dim3(nb), dim3(256), 0, s, n, ncol, Ar, Ai, ld, part);
+    hipLaunchKernelGGL(k_colnorm2_sum, dim3((ncol + 255) / 256), dim3(256), 0, s, nb, ncol, (const double*)part, out);
 }
 
 // Updates of at most this many rows go to the short-K kernel (below it the
@@ -237,8 +259,9 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
     const bool can_refine = c->has_Z64;
     // scratch: R (FP64 planar), Y and W (working precision planar), norms
     const size_t ysz = f32 ? sizeof(float) : sizeof(double);
-    const size_t bytes = 2 * cnt * sizeof(double) + 4 * cnt * ysz + 2 * (size_t)ldr * sizeof(double) + 256 +
-                         (f32 ? lu_super_floats(ncol) * sizeof(float) + 256 : 0);
+    const size_t npart = (size_t)colnorm_blocks(c->n) * ldr;  // column-norm partial sums
+    const size_t bytes = 2 * cnt * sizeof(double) + 4 * cnt * ysz + (2 * (size_t)ldr + npart) * sizeof(double) +
+                         256 + (f32 ? lu_super_floats(ncol) * sizeof(float) + 256 : 0);
     void* scr = nullptr;
     if (int rc = ctx_scratch(c, bytes, &scr)) return rc;
     double* Rr = static_cast<double*>(scr);
@@ -247,8 +270,9 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
     char* wp = yp + 2 * cnt * ysz;
     double* nb2 = reinterpret_cast<double*>(wp + 2 * cnt * ysz);
     double* nr2 = nb2 + ldr;
+    double* npt = nr2 + ldr;
     // the two-level FP32 solve's work area, 256-byte aligned
-    float* tt = f32 ? reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(nr2 + ldr) + 255) / 256 * 256) : nullptr;
+    float* tt = f32 ? reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(npt + npart) + 255) / 256 * 256) : nullptr;
     hipEvent_t e[4];
     for (auto& x : e) BEM_HIP(c, hipEventCreate(&x));
     const dim3 g1((unsigned)((cnt + 255) / 256));
@@ -283,7 +307,7 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
         // 3M form: three real products (see k_cgemm, ROLE 2)
         if (int rc = cgemm_splitk_ctx<double>(c, n, ncol, n, c->Zr, c->Zi, n, c->Xr, c->Xi, ldr, Rr, Ri, ldr, 0, 2))
             return rc;
-        hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, s, c->n, Rr, Ri, (long)ldr, nr2);
+        colnorm2(s, c->n, nrhs, Rr, Ri, (long)ldr, npt, nr2);
         BEM_HIP(c, hipMemcpyAsync(r2.data(), nr2, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
         BEM_HIP(c, hipStreamSynchronize(s));
         return 0;
@@ -300,7 +324,7 @@ int lu_solve_launch(bem_ctx* c, int nrhs, double tol, int maxit, int* iters, dou
     int rc = 0;
     if (can_refine && (maxit > 0 || f32)) {
         BEM_HIP(c, hipEventRecord(e[2], s));
-        hipLaunchKernelGGL(k_colnorm2, dim3(nrhs), dim3(256), 0, s, c->n, c->Br, c->Bi, (long)ldr, nb2);
+        colnorm2(s, c->n, nrhs, c->Br, c->Bi, (long)ldr, npt, nb2);
         BEM_HIP(c, hipMemcpyAsync(b2.data(), nb2, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
         if (int e2 = residual()) return e2;
         *res0 = *res = max_ratio(r2, b2, nrhs);
